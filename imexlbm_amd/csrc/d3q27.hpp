@@ -1,0 +1,226 @@
+// d3q27.hpp -- D3Q27 lattice, storage formats and the BGK collision for gfx950.
+//
+// Direction order, weights and velocities are the reference's
+// (src/lbm.cpp:9-35): 0 rest, 1-6 faces, 7-18 edges, 19-26 corners, with each
+// direction followed by its opposite.  That pairing is what both collision
+// forms below exploit: e_b = -e_a makes e_b.u = -(e_a.u) exactly in IEEE
+// arithmetic, so one pair shares every product.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace imex {
+
+constexpr int Q = 27;
+
+// Velocities e[ii] = (EX, EY, EZ), src/lbm.cpp:20-30.
+constexpr int EX[Q] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0,
+                       1, -1, 1, -1, 1, -1, 1, -1};
+constexpr int EY[Q] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1,
+                       1, -1, -1, 1, 1, -1, -1, 1};
+constexpr int EZ[Q] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1,
+                       1, -1, 1, -1, -1, 1, -1, 1};
+
+// Weights as the same correctly rounded quotients, src/lbm.cpp:10-17.
+constexpr double W_REST = 8.0 / 27.0;
+constexpr double W_FACE = 2.0 / 27.0;
+constexpr double W_EDGE = 1.0 / 54.0;
+constexpr double W_CORNER = 1.0 / 216.0;
+
+__host__ __device__ constexpr double weight(int ii)
+{
+    return ii == 0 ? W_REST : (ii <= 6 ? W_FACE : (ii <= 18 ? W_EDGE : W_CORNER));
+}
+
+// ---------------------------------------------------------------------------
+// Storage formats.  FP64 stores f itself.  The reduced formats store the
+// shifted population g = f - w*rho0 and compute in FP32.
+// ---------------------------------------------------------------------------
+struct bf16_t { uint16_t bits; };
+
+template <class S> struct Store;
+
+template <> struct Store<double> {
+    using real = double;
+    static constexpr bool shifted = false;
+    __device__ static double ld(const double *p) { return *p; }
+    template <bool NT> __device__ static void st(double *p, double v)
+    {
+        if constexpr (NT) __builtin_nontemporal_store(v, p); else *p = v;
+    }
+};
+
+template <> struct Store<float> {
+    using real = float;
+    static constexpr bool shifted = true;
+    __device__ static float ld(const float *p) { return *p; }
+    template <bool NT> __device__ static void st(float *p, float v)
+    {
+        if constexpr (NT) __builtin_nontemporal_store(v, p); else *p = v;
+    }
+};
+
+template <> struct Store<_Float16> {
+    using real = float;
+    static constexpr bool shifted = true;
+    __device__ static float ld(const _Float16 *p) { return (float)*p; }
+    template <bool NT> __device__ static void st(_Float16 *p, float v)
+    {
+        const _Float16 h = (_Float16)v;   // round to nearest even
+        if constexpr (NT) __builtin_nontemporal_store(h, p); else *p = h;
+    }
+};
+
+// bfloat16 as raw bits: widening is a shift, narrowing rounds to nearest
+// even (NaN kept quiet).
+__host__ __device__ inline float bf16_to_float(uint16_t b)
+{
+    union { uint32_t u; float f; } c;
+    c.u = (uint32_t)b << 16;
+    return c.f;
+}
+__host__ __device__ inline uint16_t float_to_bf16(float f)
+{
+    union { uint32_t u; float f; } c;
+    c.f = f;
+    if ((c.u & 0x7f800000u) == 0x7f800000u && (c.u & 0x007fffffu))
+        return (uint16_t)((c.u >> 16) | 0x40u);
+    const uint32_t r = c.u + 0x7fffu + ((c.u >> 16) & 1u);
+    return (uint16_t)(r >> 16);
+}
+
+template <> struct Store<bf16_t> {
+    using real = float;
+    static constexpr bool shifted = true;
+    __device__ static float ld(const bf16_t *p)
+    {
+        return bf16_to_float(*reinterpret_cast<const uint16_t *>(p));
+    }
+    template <bool NT> __device__ static void st(bf16_t *p, float v)
+    {
+        const uint16_t b = float_to_bf16(v);
+        uint16_t *q = reinterpret_cast<uint16_t *>(p);
+        if constexpr (NT) __builtin_nontemporal_store(b, q); else *q = b;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// FP64 collision in the reference's operation order (src/lbm.cpp:140-164),
+// compiled without contraction.  Moments skip the e = 0 terms: adding
+// f*0 = +-0 leaves a sum unchanged except for the sign of an exact zero, which
+// no later non-zero result can see.  Sums run in direction order 0..26.
+// ---------------------------------------------------------------------------
+struct Moments64 { double pl, ul, vl, wl; };
+
+__device__ inline Moments64 moments64(const double (&f)[Q])
+{
+    double pl = 0.0, ul = 0.0, vl = 0.0, wl = 0.0;
+#pragma unroll
+    for (int ii = 0; ii < Q; ++ii) {
+        pl += f[ii];
+        if (EX[ii] == 1) ul += f[ii]; else if (EX[ii] == -1) ul -= f[ii];
+        if (EY[ii] == 1) vl += f[ii]; else if (EY[ii] == -1) vl -= f[ii];
+        if (EZ[ii] == 1) wl += f[ii]; else if (EZ[ii] == -1) wl -= f[ii];
+    }
+    return {pl, ul, vl, wl};
+}
+
+// feq for the pair (a, a+1) with e_a.u = s and e_{a+1}.u = -s, as the
+// reference writes it: w*(3*pp + 3*edu + 4.5*edu*edu - 1.5*udu), left to right.
+__device__ inline void feq_pair64(double p3, double y, double s, double w,
+                                  double &fa, double &fb)
+{
+    const double s3 = 3.0 * s;
+    const double x = (4.5 * s) * s;
+    fa = w * (((p3 + s3) + x) - y);
+    fb = w * (((p3 - s3) + x) - y);
+}
+
+// The 13 pair projections e_a.u, each with the reference's association
+// ((e0*u + e1*v) + e2*w).
+__device__ inline void pair_proj(double u, double v, double w, double (&s)[13])
+{
+    s[0] = u;            s[1] = v;            s[2] = w;
+    s[3] = u + v;        s[4] = u - v;        s[5] = u + w;
+    s[6] = u - w;        s[7] = v + w;        s[8] = v - w;
+    s[9] = (u + v) + w;  s[10] = (u - v) + w; s[11] = (u + v) - w;
+    s[12] = (u - v) - w;
+}
+
+__device__ inline void collide64(double (&f)[Q], double inv_tau)
+{
+    const Moments64 m = moments64(f);
+    const double pp = m.pl * (1.0 / 3.0);
+    const double udu = m.ul * m.ul + m.vl * m.vl + m.wl * m.wl;
+    const double p3 = 3.0 * pp;
+    const double y = 1.5 * udu;
+    double s[13];
+    pair_proj(m.ul, m.vl, m.wl, s);
+    {
+        const double feq = W_REST * (p3 - y);
+        f[0] = f[0] - (f[0] - feq) * inv_tau;
+    }
+#pragma unroll
+    for (int pr = 0; pr < 13; ++pr) {
+        const int a = 1 + 2 * pr, b = a + 1;
+        double fa, fb;
+        feq_pair64(p3, y, s[pr], weight(a), fa, fb);
+        f[a] = f[a] - (f[a] - fa) * inv_tau;
+        f[b] = f[b] - (f[b] - fb) * inv_tau;
+    }
+}
+
+// Initial equilibrium (src/lbm.cpp:101-106): the same expression with the
+// analytic pressure p in place of pp and w = 0.
+__device__ inline void equilibrium64(double p, double u, double v, double (&f)[Q])
+{
+    const double w = 0.0;
+    const double udu = u * u + v * v + w * w;
+    const double p3 = 3.0 * p;
+    const double y = 1.5 * udu;
+    double s[13];
+    pair_proj(u, v, w, s);
+    f[0] = W_REST * (p3 - y);
+#pragma unroll
+    for (int pr = 0; pr < 13; ++pr) {
+        const int a = 1 + 2 * pr;
+        feq_pair64(p3, y, s[pr], weight(a), f[a], f[a + 1]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP32 collision on shifted populations g = f - w*rho0.  With Σw = 1 and
+// Σw e = 0: Σf = Σg + rho0 and Σf e = Σg e, so the shifted equilibrium is
+// geq = w*(Σg + 3 e.u + 4.5 (e.u)^2 - 1.5 u.u) and rho0 drops out.  The update
+// g' = (1-ω) g + ω geq is written with fused multiply-adds; moments are built
+// from the three x-planes so partial sums are shared.
+// ---------------------------------------------------------------------------
+__device__ inline void collide32(float (&g)[Q], float omega)
+{
+    const float xp = (((g[1] + g[7]) + (g[9] + g[11])) + ((g[13] + g[19]) + (g[21] + g[23]))) + g[25];
+    const float xm = (((g[2] + g[8]) + (g[10] + g[12])) + ((g[14] + g[20]) + (g[22] + g[24]))) + g[26];
+    const float x0 = (((g[0] + g[3]) + (g[4] + g[5])) + ((g[6] + g[15]) + (g[16] + g[17]))) + g[18];
+    const float yp = (((g[3] + g[7]) + (g[10] + g[15])) + ((g[17] + g[19]) + (g[22] + g[23]))) + g[26];
+    const float ym = (((g[4] + g[8]) + (g[9] + g[16])) + ((g[18] + g[20]) + (g[21] + g[24]))) + g[25];
+    const float zp = (((g[5] + g[11]) + (g[14] + g[15])) + ((g[18] + g[19]) + (g[21] + g[24]))) + g[26];
+    const float zm = (((g[6] + g[12]) + (g[13] + g[16])) + ((g[17] + g[20]) + (g[22] + g[23]))) + g[25];
+    const float sig = (xp + xm) + x0;
+    const float u = xp - xm, v = yp - ym, w = zp - zm;
+    const float udu = __builtin_fmaf(u, u, __builtin_fmaf(v, v, w * w));
+    const float A = __builtin_fmaf(-1.5f, udu, sig);
+    const float om1 = 1.0f - omega;
+    g[0] = __builtin_fmaf(om1, g[0], (omega * (float)W_REST) * A);
+    const float s[13] = {u, v, w, u + v, u - v, u + w, u - w, v + w, v - w,
+                         (u + v) + w, (u - v) + w, (u + v) - w, (u - v) - w};
+#pragma unroll
+    for (int pr = 0; pr < 13; ++pr) {
+        const int a = 1 + 2 * pr, b = a + 1;
+        const float ow = omega * (float)weight(a);
+        const float B = __builtin_fmaf(4.5f * s[pr], s[pr], A) * ow;
+        const float C = (3.0f * s[pr]) * ow;
+        g[a] = __builtin_fmaf(om1, g[a], B + C);
+        g[b] = __builtin_fmaf(om1, g[b], B - C);
+    }
+}
+
+}  // namespace imex

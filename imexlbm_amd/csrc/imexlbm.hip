@@ -1,0 +1,986 @@
+// imexlbm.hip -- context, device memory, launch sequencing and the C ABI.
+//
+// The reference's step is Collision() (in place) then Streaming() (ghost fill,
+// pull into ft, swap), src/lbm.cpp:131-207.  Here the context holds either
+// the post-streaming state S (as the reference does) or a post-collision
+// state P whose pull-stream is still owed.  From P one fused launch does
+// "pull-stream, then collide" and yields the next P, so each step reads and
+// writes the populations exactly once.  Anything that observes S (the
+// conserved sums, a download) gathers it through the pull offsets, which is
+// bit-identical to materialising it because streaming is a copy.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/imexlbm.h"
+#include "kernels.hpp"
+
+using namespace imex;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPC(call)                                                                        \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return fail(IMEX_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));    \
+    } while (0)
+
+#define TRY(call)                      \
+    do {                               \
+        int r_ = (call);               \
+        if (r_ != IMEX_OK) return r_;  \
+    } while (0)
+
+constexpr double TGV_PI = 3.1415926;  // the literal the reference uses, src/lbm.cpp:3
+
+int storage_bytes(int prec)
+{
+    switch (prec) {
+    case IMEX_F64: return 8;
+    case IMEX_F32: return 4;
+    case IMEX_F16: return 2;
+    case IMEX_BF16: return 2;
+    default: return 0;
+    }
+}
+
+long long round_up(long long v, long long a) { return (v + a - 1) / a * a; }
+
+struct Instance {
+    imex_domain dom;
+    double omega;
+    int box_begin, box_end;
+    int tile_begin, tile_end;
+    int tab[3];
+};
+
+// LBM::setup_Cartesian's factorisation (src/mpi_view_transfer.cpp:55-91): the
+// first strict minimum of the spread of the int local extents, x outer.
+void cartesian(int n, const int gl[3], int r[3])
+{
+    double min_e = 1e16;
+    r[0] = r[1] = r[2] = 1;
+    for (int i = 1; i <= n; ++i) {
+        if (n % i) continue;
+        for (int j = 1; j <= n / i; ++j) {
+            if ((n / i) % j) continue;
+            const int k = n / i / j;
+            const double a = gl[0] / i, b = gl[1] / j, c = gl[2] / k;
+            const double m = (a + b + c) / 3.0;
+            const double e = (a - m) * (a - m) + (b - m) * (b - m) + (c - m) * (c - m);
+            if (e < min_e) {
+                min_e = e;
+                r[0] = i; r[1] = j; r[2] = k;
+            }
+        }
+    }
+}
+
+// setup_Local's block distribution (src/mpi_view_transfer.cpp:165-188).
+int block_len(int gl, int r, int p) { return (p - gl % r >= 0) ? gl / r : gl / r + 1; }
+int block_lo(int gl, int r, int p) { return p * (gl / r) + std::min(p, gl % r); }
+
+}  // namespace
+
+struct imex_ctx {
+    imex_config cfg{};
+    int prec = IMEX_F64;
+    int ssize = 8;
+    bool tiled = false;
+    int device = 0;
+    int ncu = 256;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+
+    std::vector<Instance> inst;
+    std::vector<BoxDesc> boxes;
+    std::vector<int> box_coord;  // tile coordinates, 3 per box (tiled mode)
+    std::vector<WorkTile> tiles;
+    std::vector<HaloSeg> segs;
+    std::vector<int2> pack_blocks, unpack_blocks;
+    int rgrid[3] = {1, 1, 1};
+
+    BoxDesc *d_boxes = nullptr;
+    WorkTile *d_tiles = nullptr;
+    size_t d_tiles_cap = 0;
+    HaloSeg *d_segs = nullptr;
+    int2 *d_pack = nullptr, *d_unpack = nullptr;
+    double *d_tabs = nullptr;
+    double *d_partial = nullptr;
+    double *d_out = nullptr;
+    double *h_out = nullptr;
+    double *d_stage = nullptr;
+    size_t stage_cap = 0;
+    void *pool[2] = {nullptr, nullptr};
+    long long pool_elems = 0;
+    void *xbuf = nullptr;
+    long long xbuf_elems = 0;
+
+    int cur = 0;
+    bool initialized = false;
+    bool state_S = true;       // pool[cur] holds S (true) or P with a pull owed
+    bool pending = false;      // a Collision() not yet applied
+    bool ghosts_valid = false; // tiled: ghost layers of pool[cur] are current
+
+    int block = 256, cpt = 1, persistent = 0, nt = 0, dir_pad = 0, halo_depth = 1;
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+
+    long long cells = 0;
+    long long device_bytes = 0;
+};
+
+namespace {
+
+void *pool_ptr(imex_ctx *c, int which) { return c->pool[which]; }
+
+// --------------------------------------------------------------------------
+// Geometry
+// --------------------------------------------------------------------------
+int layout_boxes(imex_ctx *c)
+{
+    const long long align = 128 / c->ssize;   // elements per 128 bytes
+    c->boxes.clear();
+    c->box_coord.clear();
+    long long start = 0;
+    c->cells = 0;
+    for (size_t in = 0; in < c->inst.size(); ++in) {
+        Instance &I = c->inst[in];
+        const int gl[3] = {I.dom.nx, I.dom.ny, I.dom.nz};
+        int r[3] = {1, 1, 1};
+        if (c->tiled) {
+            cartesian(c->cfg.ntiles, gl, r);
+            std::copy(r, r + 3, c->rgrid);
+        }
+        I.box_begin = (int)c->boxes.size();
+        const int nb = r[0] * r[1] * r[2];
+        for (int me = 0; me < nb; ++me) {
+            // rank order me = px + rx*pz + rx*rz*py (src/mpi_view_transfer.cpp:96-98)
+            const int p[3] = {me % r[0], me / r[0] / r[2], (me / r[0]) % r[2]};
+            BoxDesc b{};
+            int n[3];
+            for (int a = 0; a < 3; ++a) {
+                n[a] = block_len(gl[a], r[a], p[a]);
+                b.lo[a] = block_lo(gl[a], r[a], p[a]);
+                b.gl[a] = gl[a];
+                b.tab[a] = I.tab[a];
+                if (n[a] < 1) return fail(IMEX_EINVAL, "more tiles than cells along an axis");
+                if (c->tiled && n[a] < c->halo_depth)
+                    return fail(IMEX_EINVAL, "tile thinner than the halo depth");
+            }
+            b.nx = n[0]; b.ny = n[1]; b.nz = n[2];
+            long long vol, org;
+            if (c->tiled) {
+                const long long xoff = std::max<long long>(align, 3);
+                b.sy = (int)round_up(xoff + n[0] + 3, align);
+                b.sz = b.sy * (n[1] + 6);
+                vol = (long long)b.sz * (n[2] + 6);
+                org = xoff + 3LL * b.sy + 3LL * b.sz;
+            } else {
+                b.sy = n[0];
+                b.sz = n[0] * n[1];
+                vol = (long long)b.sz * n[2];
+                org = 0;
+            }
+            if ((long long)b.sz * (n[2] + 6) >= (1LL << 31) ||
+                (long long)n[0] * n[1] * (long long)n[0] >= (1LL << 32))
+                return fail(IMEX_ERANGE, "box too large for 32-bit in-plane indexing");
+            b.dstride = round_up(vol, align) + std::max(c->dir_pad, 0);
+            b.org = start + org;
+            b.mag = ((1ULL << 32) + (unsigned long long)n[0] - 1) / (unsigned long long)n[0];
+            b.inst = (int)in;
+            b.omega = I.omega;
+            b.omega_f = (float)I.omega;
+            b.rho0 = I.dom.rho0;
+            b.u0 = I.dom.u0;
+            start += round_up(27LL * b.dstride, align * 2);
+            c->boxes.push_back(b);
+            c->box_coord.insert(c->box_coord.end(), p, p + 3);
+            c->cells += (long long)n[0] * n[1] * n[2];
+        }
+        I.box_end = (int)c->boxes.size();
+    }
+    c->pool_elems = start;
+    return IMEX_OK;
+}
+
+void build_tiles(imex_ctx *c)
+{
+    c->tiles.clear();
+    const int chunk = c->block * c->cpt;
+    for (Instance &I : c->inst) {
+        I.tile_begin = (int)c->tiles.size();
+        for (int bi = I.box_begin; bi < I.box_end; ++bi) {
+            const BoxDesc &b = c->boxes[bi];
+            const int plane = b.nx * b.ny;
+            for (int z = 0; z < b.nz; ++z)
+                for (int p0 = 0; p0 < plane; p0 += chunk)
+                    c->tiles.push_back({bi, z, p0, std::min(chunk, plane - p0)});
+        }
+        I.tile_end = (int)c->tiles.size();
+    }
+}
+
+int upload_tiles(imex_ctx *c)
+{
+    const size_t need = c->tiles.size() * sizeof(WorkTile);
+    if (need > c->d_tiles_cap) {
+        if (c->d_tiles) HIPC(hipFree(c->d_tiles));
+        c->d_tiles = nullptr;
+        HIPC(hipMalloc(&c->d_tiles, need));
+        c->d_tiles_cap = need;
+    }
+    HIPC(hipMemcpy(c->d_tiles, c->tiles.data(), need, hipMemcpyHostToDevice));
+    return IMEX_OK;
+}
+
+// Directions that cross from a box into its neighbour at offset ch: every
+// non-zero component of e equals that of ch.
+unsigned channel_mask(const int ch[3])
+{
+    unsigned m = 0;
+    for (int ii = 0; ii < Q; ++ii) {
+        const int e[3] = {EX[ii], EY[ii], EZ[ii]};
+        bool ok = true;
+        for (int a = 0; a < 3; ++a)
+            if (ch[a] != 0 && e[a] != ch[a]) ok = false;
+        if (ok) m |= 1u << ii;
+    }
+    return m;
+}
+
+// Halo plan for the tiled mode: for every tile and each of its 26 neighbour
+// channels (src/mpi_view_transfer.cpp:260-449), one pack segment (boundary
+// slab -> exchange buffer) and one unpack segment (exchange buffer -> the
+// neighbour's ghost slab on the opposite side).  Only the directions that
+// cross the channel are carried (9 per face, 3 per edge, 1 per corner).
+void build_halo(imex_ctx *c)
+{
+    c->segs.clear();
+    c->pack_blocks.clear();
+    c->unpack_blocks.clear();
+    const int d = c->halo_depth;
+    long long xoff = 0;
+    std::vector<HaloSeg> unpack;
+    for (const Instance &I : c->inst) {
+        const int nb = I.box_end - I.box_begin;
+        for (int bi = I.box_begin; bi < I.box_end; ++bi) {
+            const BoxDesc &b = c->boxes[bi];
+            const int *p = &c->box_coord[3 * bi];
+            for (int cx = -1; cx <= 1; ++cx)
+                for (int cy = -1; cy <= 1; ++cy)
+                    for (int cz = -1; cz <= 1; ++cz) {
+                        if (!cx && !cy && !cz) continue;
+                        const int ch[3] = {cx, cy, cz};
+                        int q[3];
+                        for (int a = 0; a < 3; ++a) q[a] = (p[a] + ch[a] + c->rgrid[a]) % c->rgrid[a];
+                        const int me = q[0] + c->rgrid[0] * q[2] + c->rgrid[0] * c->rgrid[2] * q[1];
+                        if (me >= nb) continue;
+                        const BoxDesc &t = c->boxes[I.box_begin + me];
+                        const int n[3] = {b.nx, b.ny, b.nz};
+                        const int nt[3] = {t.nx, t.ny, t.nz};
+                        int s0[3], d0[3], ext[3];
+                        for (int a = 0; a < 3; ++a) {
+                            if (ch[a] == 0) { s0[a] = 0; d0[a] = 0; ext[a] = n[a]; }
+                            else if (ch[a] > 0) { s0[a] = n[a] - d; d0[a] = -d; ext[a] = d; }
+                            else { s0[a] = 0; d0[a] = nt[a]; ext[a] = d; }
+                        }
+                        const unsigned mask = channel_mask(ch);
+                        const int ndir = __builtin_popcount(mask);
+                        HaloSeg ps{};
+                        ps.ex = ext[0]; ps.ey = ext[1]; ps.ez = ext[2];
+                        ps.ncell = ext[0] * ext[1] * ext[2];
+                        ps.mask = mask;
+                        ps.pack = 1;
+                        ps.s_off = b.org + s0[0] + (long long)s0[1] * b.sy + (long long)s0[2] * b.sz;
+                        ps.s_ds = b.dstride;
+                        ps.s_sy = b.sy; ps.s_sz = b.sz;
+                        ps.d_off = xoff;
+                        ps.d_ds = ps.ncell;
+                        ps.d_sy = ext[0]; ps.d_sz = ext[0] * ext[1];
+                        HaloSeg us = ps;
+                        us.pack = 0;
+                        us.s_off = xoff;
+                        us.s_ds = ps.ncell;
+                        us.s_sy = ext[0]; us.s_sz = ext[0] * ext[1];
+                        us.d_off = t.org + d0[0] + (long long)d0[1] * t.sy + (long long)d0[2] * t.sz;
+                        us.d_ds = t.dstride;
+                        us.d_sy = t.sy; us.d_sz = t.sz;
+                        xoff += round_up((long long)ndir * ps.ncell, 64);
+                        c->segs.push_back(ps);
+                        unpack.push_back(us);
+                    }
+        }
+    }
+    const int npack = (int)c->segs.size();
+    for (int s = 0; s < npack; ++s)
+        for (int c0 = 0; c0 < c->segs[s].ncell; c0 += 256) c->pack_blocks.push_back({s, c0});
+    for (int s = 0; s < (int)unpack.size(); ++s)
+        for (int c0 = 0; c0 < unpack[s].ncell; c0 += 256) c->unpack_blocks.push_back({npack + s, c0});
+    c->segs.insert(c->segs.end(), unpack.begin(), unpack.end());
+    c->xbuf_elems = xoff;
+}
+
+// --------------------------------------------------------------------------
+// Launchers
+// --------------------------------------------------------------------------
+template <class S>
+int launch_step_t(imex_ctx *c, int op, int src, int dst, int t0, int n)
+{
+    const S *sp = (const S *)pool_ptr(c, src);
+    S *dp = (S *)pool_ptr(c, dst);
+    const WorkTile *tl = c->d_tiles + t0;
+    const int grid = c->persistent > 0 ? std::min(c->persistent, n) : n;
+    const dim3 g(grid), b(c->block);
+    const bool nt = c->nt != 0;
+#define LAUNCH(W, O, N) hipLaunchKernelGGL((k_step<S, W, O, N>), g, b, 0, c->stream, sp, dp, c->d_boxes, tl, n, c->cpt)
+    if (!c->tiled) {
+        if (op == OP_FUSED) { if (nt) LAUNCH(true, OP_FUSED, true); else LAUNCH(true, OP_FUSED, false); }
+        else if (op == OP_COLLIDE) LAUNCH(true, OP_COLLIDE, false);
+        else LAUNCH(true, OP_STREAM, false);
+    } else {
+        if (op == OP_FUSED) { if (nt) LAUNCH(false, OP_FUSED, true); else LAUNCH(false, OP_FUSED, false); }
+        else if (op == OP_COLLIDE) LAUNCH(false, OP_COLLIDE, false);
+        else LAUNCH(false, OP_STREAM, false);
+    }
+#undef LAUNCH
+    HIPC(hipGetLastError());
+    return IMEX_OK;
+}
+
+int launch_step(imex_ctx *c, int op, int src, int dst)
+{
+    auto one = [&](int t0, int n) -> int {
+        if (n <= 0) return IMEX_OK;
+        switch (c->prec) {
+        case IMEX_F64: return launch_step_t<double>(c, op, src, dst, t0, n);
+        case IMEX_F32: return launch_step_t<float>(c, op, src, dst, t0, n);
+        case IMEX_F16: return launch_step_t<_Float16>(c, op, src, dst, t0, n);
+        default: return launch_step_t<bf16_t>(c, op, src, dst, t0, n);
+        }
+    };
+    if (c->cfg.sequential && c->inst.size() > 1) {
+        for (const Instance &I : c->inst) TRY(one(I.tile_begin, I.tile_end - I.tile_begin));
+        return IMEX_OK;
+    }
+    return one(0, (int)c->tiles.size());
+}
+
+template <class S>
+int launch_halo_t(imex_ctx *c, int which)
+{
+    S *f = (S *)pool_ptr(c, which);
+    S *x = (S *)c->xbuf;
+    const int np = (int)c->pack_blocks.size(), nu = (int)c->unpack_blocks.size();
+    if (np) hipLaunchKernelGGL((k_halo<S>), dim3(np), dim3(256), 0, c->stream, f, x, c->d_segs, c->d_pack, np);
+    if (nu) hipLaunchKernelGGL((k_halo<S>), dim3(nu), dim3(256), 0, c->stream, x, f, c->d_segs, c->d_unpack, nu);
+    HIPC(hipGetLastError());
+    return IMEX_OK;
+}
+
+int ensure_ghosts(imex_ctx *c)
+{
+    if (!c->tiled || c->ghosts_valid) return IMEX_OK;
+    switch (c->prec) {
+    case IMEX_F64: TRY(launch_halo_t<double>(c, c->cur)); break;
+    case IMEX_F32: TRY(launch_halo_t<float>(c, c->cur)); break;
+    case IMEX_F16: TRY(launch_halo_t<_Float16>(c, c->cur)); break;
+    default: TRY(launch_halo_t<bf16_t>(c, c->cur)); break;
+    }
+    c->ghosts_valid = true;
+    return IMEX_OK;
+}
+
+int do_collide(imex_ctx *c)
+{
+    TRY(launch_step(c, OP_COLLIDE, c->cur, c->cur));
+    c->ghosts_valid = false;
+    return IMEX_OK;
+}
+
+int do_fused(imex_ctx *c)
+{
+    TRY(ensure_ghosts(c));
+    TRY(launch_step(c, OP_FUSED, c->cur, 1 - c->cur));
+    c->cur = 1 - c->cur;
+    c->ghosts_valid = false;
+    return IMEX_OK;
+}
+
+int do_stream(imex_ctx *c)
+{
+    TRY(ensure_ghosts(c));
+    TRY(launch_step(c, OP_STREAM, c->cur, 1 - c->cur));
+    c->cur = 1 - c->cur;
+    c->ghosts_valid = false;
+    return IMEX_OK;
+}
+
+// Apply a deferred Collision() whose Streaming() has not come: the logical
+// state becomes the collided one, stored as S.
+int flush_pending(imex_ctx *c)
+{
+    if (!c->pending) return IMEX_OK;
+    if (c->state_S) TRY(do_collide(c));
+    else TRY(do_fused(c));
+    c->state_S = true;
+    c->pending = false;
+    return IMEX_OK;
+}
+
+int one_step(imex_ctx *c)
+{
+    TRY(flush_pending(c));
+    if (c->state_S) TRY(do_collide(c));
+    else TRY(do_fused(c));
+    c->state_S = false;
+    return IMEX_OK;
+}
+
+void drop_graphs(imex_ctx *c)
+{
+    for (int k = 0; k < 2; ++k) {
+        if (c->gexec[k]) (void)hipGraphExecDestroy(c->gexec[k]);
+        if (c->graph[k]) (void)hipGraphDestroy(c->graph[k]);
+        c->gexec[k] = nullptr;
+        c->graph[k] = nullptr;
+    }
+}
+
+// Two fused steps starting from pool[parity], captured once and replayed.
+int build_graph(imex_ctx *c, int parity)
+{
+    const int save_cur = c->cur;
+    const bool save_gv = c->ghosts_valid;
+    c->cur = parity;
+    c->ghosts_valid = false;
+    HIPC(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int r = do_fused(c);
+    if (r == IMEX_OK) r = do_fused(c);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    c->cur = save_cur;
+    c->ghosts_valid = save_gv;
+    if (r != IMEX_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return r;
+    }
+    if (e != hipSuccess) return fail(IMEX_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    c->graph[parity] = g;
+    HIPC(hipGraphInstantiate(&c->gexec[parity], g, nullptr, nullptr, 0));
+    return IMEX_OK;
+}
+
+int run_steps(imex_ctx *c, int nsteps)
+{
+    while (nsteps > 0) {
+        if (c->cfg.use_graph && nsteps >= 2 && !c->state_S && !c->pending) {
+            if (!c->gexec[c->cur]) TRY(build_graph(c, c->cur));
+            HIPC(hipGraphLaunch(c->gexec[c->cur], c->stream));
+            c->ghosts_valid = false;
+            nsteps -= 2;
+            continue;
+        }
+        TRY(one_step(c));
+        --nsteps;
+    }
+    return IMEX_OK;
+}
+
+template <class S>
+int launch_conserved_t(imex_ctx *c, int t0, int n, bool gather)
+{
+    const S *sp = (const S *)pool_ptr(c, c->cur);
+    const int grid = std::min(n, 1024);
+    const WorkTile *tl = c->d_tiles + t0;
+    if (!c->tiled) {
+        if (gather) hipLaunchKernelGGL((k_conserved<S, true, true>), dim3(grid), dim3(RED_BLOCK), 0, c->stream, sp, c->d_boxes, tl, n, c->cpt, c->d_partial);
+        else hipLaunchKernelGGL((k_conserved<S, true, false>), dim3(grid), dim3(RED_BLOCK), 0, c->stream, sp, c->d_boxes, tl, n, c->cpt, c->d_partial);
+    } else {
+        if (gather) hipLaunchKernelGGL((k_conserved<S, false, true>), dim3(grid), dim3(RED_BLOCK), 0, c->stream, sp, c->d_boxes, tl, n, c->cpt, c->d_partial);
+        else hipLaunchKernelGGL((k_conserved<S, false, false>), dim3(grid), dim3(RED_BLOCK), 0, c->stream, sp, c->d_boxes, tl, n, c->cpt, c->d_partial);
+    }
+    hipLaunchKernelGGL(k_conserved_final, dim3(1), dim3(RED_BLOCK), 0, c->stream, c->d_partial, grid, c->d_out);
+    HIPC(hipGetLastError());
+    return IMEX_OK;
+}
+
+template <class S>
+int launch_export_t(imex_ctx *c, int box, bool gather)
+{
+    const S *sp = (const S *)pool_ptr(c, c->cur);
+    const BoxDesc &b = c->boxes[box];
+    const long long ncell = (long long)b.nx * b.ny * b.nz;
+    const int grid = (int)std::min<long long>((ncell + 255) / 256, 65536);
+    if (!c->tiled) {
+        if (gather) hipLaunchKernelGGL((k_export<S, true, true>), dim3(grid), dim3(256), 0, c->stream, sp, c->d_boxes, box, c->d_stage);
+        else hipLaunchKernelGGL((k_export<S, true, false>), dim3(grid), dim3(256), 0, c->stream, sp, c->d_boxes, box, c->d_stage);
+    } else {
+        if (gather) hipLaunchKernelGGL((k_export<S, false, true>), dim3(grid), dim3(256), 0, c->stream, sp, c->d_boxes, box, c->d_stage);
+        else hipLaunchKernelGGL((k_export<S, false, false>), dim3(grid), dim3(256), 0, c->stream, sp, c->d_boxes, box, c->d_stage);
+    }
+    HIPC(hipGetLastError());
+    return IMEX_OK;
+}
+
+template <class S>
+int launch_import_t(imex_ctx *c, int box)
+{
+    S *dp = (S *)pool_ptr(c, c->cur);
+    const BoxDesc &b = c->boxes[box];
+    const long long ncell = (long long)b.nx * b.ny * b.nz;
+    const int grid = (int)std::min<long long>((ncell + 255) / 256, 65536);
+    hipLaunchKernelGGL((k_import<S>), dim3(grid), dim3(256), 0, c->stream, dp, c->d_boxes, box, c->d_stage);
+    HIPC(hipGetLastError());
+    return IMEX_OK;
+}
+
+template <class S>
+int launch_init_t(imex_ctx *c)
+{
+    S *dp = (S *)pool_ptr(c, c->cur);
+    const int n = (int)c->tiles.size();
+    hipLaunchKernelGGL((k_init<S>), dim3(std::min(n, 65536)), dim3(c->block), 0, c->stream, dp, c->d_boxes, c->d_tiles, n, c->cpt, c->d_tabs);
+    HIPC(hipGetLastError());
+    return IMEX_OK;
+}
+
+#define DISPATCH(fn, ...)                                                   \
+    (c->prec == IMEX_F64 ? fn<double>(__VA_ARGS__)                          \
+     : c->prec == IMEX_F32 ? fn<float>(__VA_ARGS__)                         \
+     : c->prec == IMEX_F16 ? fn<_Float16>(__VA_ARGS__)                      \
+                           : fn<bf16_t>(__VA_ARGS__))
+
+int ensure_stage(imex_ctx *c, size_t doubles)
+{
+    if (doubles <= c->stage_cap) return IMEX_OK;
+    if (c->d_stage) HIPC(hipFree(c->d_stage));
+    c->d_stage = nullptr;
+    c->stage_cap = 0;
+    if (hipMalloc(&c->d_stage, doubles * sizeof(double)) != hipSuccess)
+        return fail(IMEX_ENOMEM, "staging buffer allocation failed");
+    c->stage_cap = doubles;
+    return IMEX_OK;
+}
+
+int check_ctx(imex_ctx *c)
+{
+    if (!c) return fail(IMEX_EINVAL, "null context");
+    HIPC(hipSetDevice(c->device));
+    return IMEX_OK;
+}
+
+int check_domain(imex_ctx *c, int domain)
+{
+    if (domain < 0 || domain >= (int)c->inst.size()) return fail(IMEX_EINVAL, "domain index out of range");
+    if (!c->initialized) return fail(IMEX_ESTATE, "imex_initialize has not been called");
+    return IMEX_OK;
+}
+
+}  // namespace
+
+// ==========================================================================
+// C ABI
+// ==========================================================================
+extern "C" {
+
+void imex_config_default(imex_config *cfg)
+{
+    if (!cfg) return;
+    std::memset(cfg, 0, sizeof(*cfg));
+    cfg->precision = IMEX_F64;
+    cfg->mode = IMEX_WRAP;
+    cfg->ntiles = 8;
+    cfg->halo_depth = 1;
+    cfg->dir_pad = -1;
+}
+
+const char *imex_strerror(int s)
+{
+    switch (s) {
+    case IMEX_OK: return "ok";
+    case IMEX_EINVAL: return "invalid argument";
+    case IMEX_ENOMEM: return "out of memory";
+    case IMEX_EHIP: return "HIP runtime error";
+    case IMEX_ENODEV: return "no GPU";
+    case IMEX_ESTATE: return "call out of order";
+    case IMEX_ERANGE: return "size out of range";
+    default: return "unknown status";
+    }
+}
+
+const char *imex_last_error(void) { return g_err.c_str(); }
+
+int imex_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+void imex_destroy(imex_ctx *c)
+{
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    drop_graphs(c);
+    for (void *p : {(void *)c->d_boxes, (void *)c->d_tiles, (void *)c->d_segs, (void *)c->d_pack,
+                    (void *)c->d_unpack, (void *)c->d_tabs, (void *)c->d_partial, (void *)c->d_out,
+                    (void *)c->d_stage, c->pool[0], c->pool[1], c->xbuf})
+        if (p) (void)hipFree(p);
+    if (c->h_out) (void)hipHostFree(c->h_out);
+    for (hipEvent_t e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int imex_create(imex_ctx **out, const imex_config *cfg_in, const imex_domain *doms, int ndomains)
+{
+    if (!out || !doms || ndomains < 1) return fail(IMEX_EINVAL, "null output, null domains or ndomains < 1");
+    *out = nullptr;
+    imex_config cfg;
+    imex_config_default(&cfg);
+    if (cfg_in) cfg = *cfg_in;
+    if (storage_bytes(cfg.precision) == 0) return fail(IMEX_EINVAL, "unknown precision");
+    if (cfg.mode != IMEX_WRAP && cfg.mode != IMEX_TILED) return fail(IMEX_EINVAL, "unknown mode");
+    if (cfg.mode == IMEX_TILED && ndomains != 1) return fail(IMEX_EINVAL, "tiled mode takes one domain");
+    if (cfg.ntiles <= 0) cfg.ntiles = 8;
+    if (cfg.halo_depth <= 0) cfg.halo_depth = 1;
+    if (cfg.halo_depth > 3) return fail(IMEX_EINVAL, "halo_depth must be 1..3");
+    if (cfg.block != 0 && (cfg.block % 64 != 0 || cfg.block > 512)) return fail(IMEX_EINVAL, "block must be a multiple of 64, at most 512");
+    if (cfg.cells_per_thread < 0 || cfg.cells_per_thread > 16) return fail(IMEX_EINVAL, "cells_per_thread must be 0..16");
+    for (int d = 0; d < ndomains; ++d) {
+        const imex_domain &D = doms[d];
+        if (D.nx < 1 || D.ny < 1 || D.nz < 1) return fail(IMEX_EINVAL, "domain extents must be positive");
+        if (!(D.tau0 + 0.5 > 0.0) || !std::isfinite(D.tau0)) return fail(IMEX_EINVAL, "tau0 + 0.5 must be positive");
+        if (!std::isfinite(D.rho0) || !std::isfinite(D.u0)) return fail(IMEX_EINVAL, "rho0 and u0 must be finite");
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(IMEX_ENODEV, "no HIP device");
+    if (cfg.device < 0 || cfg.device >= ndev) return fail(IMEX_EINVAL, "device ordinal out of range");
+
+    imex_ctx *c = new imex_ctx();
+    c->cfg = cfg;
+    c->device = cfg.device;
+    c->prec = cfg.precision;
+    c->ssize = storage_bytes(cfg.precision);
+    c->tiled = cfg.mode == IMEX_TILED;
+    c->halo_depth = cfg.halo_depth;
+    c->block = cfg.block ? cfg.block : 256;
+    c->cpt = cfg.cells_per_thread ? cfg.cells_per_thread : 1;
+    c->persistent = std::max(cfg.persistent, 0);
+    c->nt = cfg.nontemporal ? 1 : 0;
+    c->dir_pad = cfg.dir_pad >= 0 ? cfg.dir_pad : 0;
+    auto bail = [&](int r) { imex_destroy(c); return r; };
+
+    if (hipSetDevice(c->device) != hipSuccess) return bail(fail(IMEX_EHIP, "hipSetDevice failed"));
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device) == hipSuccess && ncu > 0) c->ncu = ncu;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return bail(fail(IMEX_EHIP, "stream creation failed"));
+    for (hipEvent_t &e : c->ev)
+        if (hipEventCreate(&e) != hipSuccess) return bail(fail(IMEX_EHIP, "event creation failed"));
+
+    // instances and their init tables
+    int tab = 0;
+    for (int d = 0; d < ndomains; ++d) {
+        Instance I{};
+        I.dom = doms[d];
+        I.omega = 1.0 / (I.dom.tau0 + 0.5);   // src/lbm.cpp:133
+        const int gl[3] = {I.dom.nx, I.dom.ny, I.dom.nz};
+        for (int a = 0; a < 3; ++a) {
+            I.tab[a] = tab;
+            tab += 3 * gl[a];
+        }
+        c->inst.push_back(I);
+    }
+    int r = layout_boxes(c);
+    if (r != IMEX_OK) return bail(r);
+    build_tiles(c);
+    if (c->tiled) build_halo(c);
+
+    const size_t pool_bytes = (size_t)c->pool_elems * c->ssize;
+    for (int k = 0; k < 2; ++k)
+        if (hipMalloc(&c->pool[k], pool_bytes) != hipSuccess)
+            return bail(fail(IMEX_ENOMEM, "population pool allocation failed (" + std::to_string(pool_bytes) + " bytes)"));
+    c->device_bytes = 2LL * (long long)pool_bytes;
+    if (c->tiled) {
+        const size_t xb = std::max<long long>(c->xbuf_elems, 1) * (size_t)c->ssize;
+        if (hipMalloc(&c->xbuf, xb) != hipSuccess) return bail(fail(IMEX_ENOMEM, "exchange buffer allocation failed"));
+        c->device_bytes += (long long)xb;
+        const size_t sb = c->segs.size() * sizeof(HaloSeg);
+        if (hipMalloc(&c->d_segs, sb) != hipSuccess ||
+            hipMalloc(&c->d_pack, std::max<size_t>(c->pack_blocks.size(), 1) * sizeof(int2)) != hipSuccess ||
+            hipMalloc(&c->d_unpack, std::max<size_t>(c->unpack_blocks.size(), 1) * sizeof(int2)) != hipSuccess)
+            return bail(fail(IMEX_ENOMEM, "halo table allocation failed"));
+        if (hipMemcpy(c->d_segs, c->segs.data(), sb, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(c->d_pack, c->pack_blocks.data(), c->pack_blocks.size() * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(c->d_unpack, c->unpack_blocks.data(), c->unpack_blocks.size() * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess)
+            return bail(fail(IMEX_EHIP, "halo table upload failed"));
+    }
+    const size_t bb = c->boxes.size() * sizeof(BoxDesc);
+    if (hipMalloc(&c->d_boxes, bb) != hipSuccess || hipMalloc(&c->d_tabs, std::max(tab, 1) * sizeof(double)) != hipSuccess ||
+        hipMalloc(&c->d_partial, 1024 * 10 * sizeof(double)) != hipSuccess || hipMalloc(&c->d_out, 8 * sizeof(double)) != hipSuccess ||
+        hipHostMalloc(&c->h_out, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess)
+        return bail(fail(IMEX_ENOMEM, "descriptor allocation failed"));
+    if (hipMemcpy(c->d_boxes, c->boxes.data(), bb, hipMemcpyHostToDevice) != hipSuccess)
+        return bail(fail(IMEX_EHIP, "descriptor upload failed"));
+    r = upload_tiles(c);
+    if (r != IMEX_OK) return bail(r);
+
+    // Per-axis tables of the analytic field, evaluated with the C library
+    // exactly as init_macro does (src/lbm.cpp:81-93).
+    std::vector<double> h(std::max(tab, 1));
+    for (const Instance &I : c->inst) {
+        const int gl[3] = {I.dom.nx, I.dom.ny, I.dom.nz};
+        for (int a = 0; a < 3; ++a)
+            for (int x = 0; x < gl[a]; ++x) {
+                const double xx = (double)x / (double)gl[a] * 2.0 * TGV_PI;
+                h[I.tab[a] + x] = std::sin(xx);
+                h[I.tab[a] + gl[a] + x] = std::cos(xx);
+                h[I.tab[a] + 2 * gl[a] + x] = std::cos(2.0 * xx);
+            }
+    }
+    if (hipMemcpy(c->d_tabs, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+        return bail(fail(IMEX_EHIP, "table upload failed"));
+    *out = c;
+    return IMEX_OK;
+}
+
+int imex_initialize(imex_ctx *c)
+{
+    TRY(check_ctx(c));
+    drop_graphs(c);
+    c->cur = 0;
+    TRY(DISPATCH(launch_init_t, c));
+    c->initialized = true;
+    c->state_S = true;
+    c->pending = false;
+    c->ghosts_valid = false;
+    HIPC(hipStreamSynchronize(c->stream));
+    return IMEX_OK;
+}
+
+int imex_collision(imex_ctx *c)
+{
+    TRY(check_ctx(c));
+    if (!c->initialized) return fail(IMEX_ESTATE, "imex_initialize has not been called");
+    TRY(flush_pending(c));
+    c->pending = true;
+    return IMEX_OK;
+}
+
+int imex_streaming(imex_ctx *c)
+{
+    TRY(check_ctx(c));
+    if (!c->initialized) return fail(IMEX_ESTATE, "imex_initialize has not been called");
+    if (c->pending) {
+        c->pending = false;
+        if (c->state_S) TRY(do_collide(c));
+        else TRY(do_fused(c));
+    } else if (!c->state_S) {
+        TRY(do_stream(c));
+    }
+    c->state_S = false;
+    return IMEX_OK;
+}
+
+int imex_step(imex_ctx *c, int nsteps)
+{
+    TRY(check_ctx(c));
+    if (!c->initialized) return fail(IMEX_ESTATE, "imex_initialize has not been called");
+    if (nsteps < 0) return fail(IMEX_EINVAL, "nsteps < 0");
+    return run_steps(c, nsteps);
+}
+
+int imex_conserved(imex_ctx *c, int domain, double out[5])
+{
+    TRY(check_ctx(c));
+    TRY(check_domain(c, domain));
+    if (!out) return fail(IMEX_EINVAL, "null output");
+    TRY(flush_pending(c));
+    const bool gather = !c->state_S;
+    if (gather) TRY(ensure_ghosts(c));
+    const Instance &I = c->inst[domain];
+    TRY(DISPATCH(launch_conserved_t, c, I.tile_begin, I.tile_end - I.tile_begin, gather));
+    HIPC(hipMemcpyAsync(c->h_out, c->d_out, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    std::memcpy(out, c->h_out, 5 * sizeof(double));
+    return IMEX_OK;
+}
+
+int imex_get_f(imex_ctx *c, int domain, double *host)
+{
+    TRY(check_ctx(c));
+    TRY(check_domain(c, domain));
+    if (!host) return fail(IMEX_EINVAL, "null output");
+    TRY(flush_pending(c));
+    const bool gather = !c->state_S;
+    if (gather) TRY(ensure_ghosts(c));
+    const Instance &I = c->inst[domain];
+    const long long NX = I.dom.nx, NY = I.dom.ny, NZ = I.dom.nz;
+    const long long gcell = NX * NY * NZ;
+    std::vector<double> tmp;
+    for (int bi = I.box_begin; bi < I.box_end; ++bi) {
+        const BoxDesc &b = c->boxes[bi];
+        const long long ncell = (long long)b.nx * b.ny * b.nz;
+        TRY(ensure_stage(c, 27 * (size_t)ncell));
+        TRY(DISPATCH(launch_export_t, c, bi, gather));
+        tmp.resize(27 * (size_t)ncell);
+        HIPC(hipMemcpyAsync(tmp.data(), c->d_stage, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPC(hipStreamSynchronize(c->stream));
+        for (int ii = 0; ii < Q; ++ii)
+            for (int k = 0; k < b.nz; ++k)
+                for (int j = 0; j < b.ny; ++j)
+                    for (int i = 0; i < b.nx; ++i) {
+                        const long long X = b.lo[0] + i, Y = b.lo[1] + j, Z = b.lo[2] + k;
+                        host[ii * gcell + (X * NY + Y) * NZ + Z] =
+                            tmp[(size_t)ii * ncell + ((size_t)k * b.ny + j) * b.nx + i];
+                    }
+    }
+    return IMEX_OK;
+}
+
+int imex_set_f(imex_ctx *c, int domain, const double *host)
+{
+    TRY(check_ctx(c));
+    TRY(check_domain(c, domain));
+    if (!host) return fail(IMEX_EINVAL, "null input");
+    TRY(flush_pending(c));
+    if (!c->state_S) {
+        // materialise S for every instance before overwriting one of them
+        TRY(do_stream(c));
+        c->state_S = true;
+    }
+    drop_graphs(c);
+    const Instance &I = c->inst[domain];
+    const long long NY = I.dom.ny, NZ = I.dom.nz;
+    const long long gcell = (long long)I.dom.nx * NY * NZ;
+    std::vector<double> tmp;
+    for (int bi = I.box_begin; bi < I.box_end; ++bi) {
+        const BoxDesc &b = c->boxes[bi];
+        const long long ncell = (long long)b.nx * b.ny * b.nz;
+        TRY(ensure_stage(c, 27 * (size_t)ncell));
+        tmp.resize(27 * (size_t)ncell);
+        for (int ii = 0; ii < Q; ++ii)
+            for (int k = 0; k < b.nz; ++k)
+                for (int j = 0; j < b.ny; ++j)
+                    for (int i = 0; i < b.nx; ++i) {
+                        const long long X = b.lo[0] + i, Y = b.lo[1] + j, Z = b.lo[2] + k;
+                        tmp[(size_t)ii * ncell + ((size_t)k * b.ny + j) * b.nx + i] =
+                            host[ii * gcell + (X * NY + Y) * NZ + Z];
+                    }
+        HIPC(hipMemcpyAsync(c->d_stage, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        TRY(DISPATCH(launch_import_t, c, bi));
+        HIPC(hipStreamSynchronize(c->stream));
+    }
+    c->ghosts_valid = false;
+    return IMEX_OK;
+}
+
+int imex_synchronize(imex_ctx *c)
+{
+    TRY(check_ctx(c));
+    HIPC(hipStreamSynchronize(c->stream));
+    return IMEX_OK;
+}
+
+int imex_timed_steps(imex_ctx *c, int nsteps, double *wall_ms, double *kernel_ms)
+{
+    TRY(check_ctx(c));
+    if (!c->initialized) return fail(IMEX_ESTATE, "imex_initialize has not been called");
+    if (nsteps < 0) return fail(IMEX_EINVAL, "nsteps < 0");
+    HIPC(hipStreamSynchronize(c->stream));
+    const auto t0 = std::chrono::steady_clock::now();
+    HIPC(hipEventRecord(c->ev[0], c->stream));
+    TRY(run_steps(c, nsteps));
+    HIPC(hipEventRecord(c->ev[1], c->stream));
+    HIPC(hipEventSynchronize(c->ev[1]));
+    const auto t1 = std::chrono::steady_clock::now();
+    float ms = 0.f;
+    HIPC(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    if (wall_ms) *wall_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    if (kernel_ms) *kernel_ms = (double)ms;
+    return IMEX_OK;
+}
+
+int imex_autotune(imex_ctx *c, int reps, double *best_ms)
+{
+    TRY(check_ctx(c));
+    if (!c->initialized) return fail(IMEX_ESTATE, "imex_initialize has not been called");
+    if (reps < 1) reps = 3;
+    TRY(flush_pending(c));
+    TRY(ensure_ghosts(c));
+    HIPC(hipStreamSynchronize(c->stream));
+    struct Cand { int block, cpt, pers, nt; };
+    std::vector<Cand> cands;
+    const int pers_opts[3] = {0, c->ncu * 4, c->ncu * 8};
+    for (int blk : {128, 256, 512})
+        for (int cpt : {1, 2, 4})
+            for (int pers : pers_opts)
+                for (int nt : {0, 1}) cands.push_back({blk, cpt, pers, nt});
+    const Cand orig = {c->block, c->cpt, c->persistent, c->nt};
+    Cand best = orig;
+    double best_t = 1e300;
+    for (const Cand &k : cands) {
+        c->block = k.block; c->cpt = k.cpt; c->persistent = k.pers; c->nt = k.nt;
+        build_tiles(c);
+        TRY(upload_tiles(c));
+        // warm-up, then timed: fused launches cur -> other without swapping,
+        // so the state is untouched
+        TRY(launch_step(c, OP_FUSED, c->cur, 1 - c->cur));
+        HIPC(hipEventRecord(c->ev[0], c->stream));
+        for (int r = 0; r < reps; ++r) TRY(launch_step(c, OP_FUSED, c->cur, 1 - c->cur));
+        HIPC(hipEventRecord(c->ev[1], c->stream));
+        HIPC(hipEventSynchronize(c->ev[1]));
+        float ms = 0.f;
+        HIPC(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+        const double t = ms / reps;
+        if (t < best_t) { best_t = t; best = k; }
+    }
+    c->block = best.block; c->cpt = best.cpt; c->persistent = best.pers; c->nt = best.nt;
+    build_tiles(c);
+    TRY(upload_tiles(c));
+    drop_graphs(c);
+    if (best_ms) *best_ms = best_t;
+    return IMEX_OK;
+}
+
+int imex_query(imex_ctx *c, imex_info *info)
+{
+    if (!c || !info) return fail(IMEX_EINVAL, "null argument");
+    std::memset(info, 0, sizeof(*info));
+    info->cells = c->cells;
+    info->storage_bytes = c->ssize;
+    info->bytes_per_update = 2LL * Q * c->ssize;
+    info->device_bytes = c->device_bytes;
+    info->ndomains = (int)c->inst.size();
+    info->ntiles = (int)c->boxes.size();
+    info->work_tiles = (int)c->tiles.size();
+    info->block = c->block;
+    info->cells_per_thread = c->cpt;
+    info->persistent = c->persistent;
+    info->nontemporal = c->nt;
+    info->dir_pad = c->dir_pad;
+    info->use_graph = c->cfg.use_graph;
+    info->grid = c->tiled ? 1 : 0;
+    info->rx = c->rgrid[0]; info->ry = c->rgrid[1]; info->rz = c->rgrid[2];
+    return IMEX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,381 @@
+// kernels.hpp -- gfx950 kernels of the IMEXLBM hot path.
+//
+// Data layout in HBM: structure of arrays, one plane per direction,
+// f[ii][z][y][x] with x stride-1.  A "box" is one stored lattice block: a
+// whole periodic domain (no ghost cells, periodic wrap in the index math) or
+// one tile of a tiled domain (three ghost layers, rows padded so interior
+// rows start on a 128-byte boundary).  Two pools A and B with identical
+// layout hold the ping-pong state.
+//
+// Work is described by a table of work tiles, each a run of up to
+// block*cells_per_thread consecutive cells of one z-plane of one box.  Every
+// step kernel walks that table, so a single periodic domain, the tiles of a
+// tiled domain and a batch of independent instances of uneven size all run as
+// one launch per step.  Wavefronts own 64 consecutive x cells, so each of the
+// 27 loads and 27 stores of a wavefront is one contiguous 64-element run.
+#pragma once
+#include <utility>
+
+#include "d3q27.hpp"
+
+namespace imex {
+
+struct BoxDesc {
+    long long org;      // element offset of interior cell (0,0,0), direction 0
+    long long dstride;  // elements between direction planes
+    unsigned long long mag;  // ceil(2^32 / nx): row = (p * mag) >> 32
+    int nx, ny, nz;     // interior extents
+    int sy, sz;         // element strides of y rows and z planes
+    int inst;           // index of the instance (logical domain)
+    int lo[3];          // offset of this box inside its instance
+    int gl[3];          // instance extents
+    int tab[3];         // offsets of the instance's per-axis init tables
+    double omega;       // 1/(tau0+0.5)
+    double rho0, u0;
+    float omega_f;
+    int pad_;
+};
+
+struct WorkTile { int box, z, p0, n; };
+
+// One halo segment: a box of cells copied for a subset of directions, either
+// from a tile into the exchange buffer (pack) or back into ghost cells
+// (unpack).  On the tile side the direction plane is ii; in the buffer the
+// selected directions are numbered 0..ndir-1.
+struct HaloSeg {
+    long long s_off, d_off;     // element offset of the first cell, dir plane 0
+    long long s_ds, d_ds;       // direction strides
+    int ex, ey, ez, ncell;      // extents
+    int s_sy, s_sz, d_sy, d_sz; // row / plane strides
+    unsigned mask;              // directions carried (bit ii)
+    int pack;                   // 1: tile -> buffer, 0: buffer -> tile
+};
+
+enum { OP_COLLIDE = 0, OP_FUSED = 1, OP_STREAM = 2 };
+
+// Offsets, relative to a cell, of the cells at x-1, x+1, y-1, y+1, z-1, z+1.
+struct Nbr { int xm, xp, ym, yp, zm, zp; };
+
+template <bool WRAP>
+__device__ inline Nbr neighbours(const BoxDesc &d, int i, int j, int k)
+{
+    Nbr n;
+    if constexpr (WRAP) {
+        n.xm = (i == 0) ? d.nx - 1 : -1;
+        n.xp = (i == d.nx - 1) ? 1 - d.nx : 1;
+        n.ym = (j == 0) ? (d.ny - 1) * d.sy : -d.sy;
+        n.yp = (j == d.ny - 1) ? (1 - d.ny) * d.sy : d.sy;
+        n.zm = (k == 0) ? (d.nz - 1) * d.sz : -d.sz;
+        n.zp = (k == d.nz - 1) ? (1 - d.nz) * d.sz : d.sz;
+    } else {
+        n.xm = -1; n.xp = 1;
+        n.ym = -d.sy; n.yp = d.sy;
+        n.zm = -d.sz; n.zp = d.sz;
+    }
+    return n;
+}
+
+// Pull source of direction ii: the cell at x - e_ii.
+template <int ii>
+__device__ inline int pull_delta(const Nbr &n)
+{
+    int o = 0;
+    if constexpr (EX[ii] == 1) o += n.xm; else if constexpr (EX[ii] == -1) o += n.xp;
+    if constexpr (EY[ii] == 1) o += n.ym; else if constexpr (EY[ii] == -1) o += n.yp;
+    if constexpr (EZ[ii] == 1) o += n.zm; else if constexpr (EZ[ii] == -1) o += n.zp;
+    return o;
+}
+
+template <class S, bool GATHER, bool WRAP, int... I>
+__device__ inline void load_cell(const S *__restrict__ base, long long ds, int o,
+                                 const Nbr &n, typename Store<S>::real (&f)[Q],
+                                 std::integer_sequence<int, I...>)
+{
+    ((f[I] = Store<S>::ld(base + (long long)I * ds + o + (GATHER ? pull_delta<I>(n) : 0))), ...);
+}
+
+template <class S, bool NT, int... I>
+__device__ inline void store_cell(S *__restrict__ base, long long ds, int o,
+                                  const typename Store<S>::real (&f)[Q],
+                                  std::integer_sequence<int, I...>)
+{
+    (Store<S>::template st<NT>(base + (long long)I * ds + o, f[I]), ...);
+}
+
+// ---------------------------------------------------------------------------
+// Step kernel.  OP_COLLIDE: collide in place (src == dst).  OP_FUSED: pull
+// from src, collide, store to dst.  OP_STREAM: pull only (a pure copy).
+// ---------------------------------------------------------------------------
+template <class S, bool WRAP, int OP, bool NT>
+__global__ void __launch_bounds__(512)
+k_step(const S *__restrict__ src, S *__restrict__ dst, const BoxDesc *__restrict__ boxes,
+       const WorkTile *__restrict__ tiles, int ntiles, int cpt)
+{
+    using R = typename Store<S>::real;
+    constexpr bool GATHER = (OP != OP_COLLIDE);
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const WorkTile wt = tiles[t];
+        const BoxDesc &d = boxes[wt.box];
+        const long long plane = d.org + (long long)wt.z * d.sz;
+        const S *__restrict__ sp = src + plane;
+        S *__restrict__ dp = dst + plane;
+        for (int c = 0; c < cpt; ++c) {
+            const int q = c * (int)blockDim.x + (int)threadIdx.x;
+            if (q >= wt.n) break;
+            const int p = wt.p0 + q;
+            const int j = (int)(((unsigned long long)(unsigned)p * d.mag) >> 32);
+            const int i = p - j * d.nx;
+            const int o = i + j * d.sy;
+            const Nbr n = neighbours<WRAP>(d, i, j, wt.z);
+            R f[Q];
+            load_cell<S, GATHER, WRAP>(sp, d.dstride, o, n, f, std::make_integer_sequence<int, Q>{});
+            if constexpr (OP != OP_STREAM) {
+                if constexpr (Store<S>::shifted) collide32(f, d.omega_f);
+                else collide64(f, d.omega);
+            }
+            store_cell<S, NT>(dp, d.dstride, o, f, std::make_integer_sequence<int, Q>{});
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Initial state (src/lbm.cpp:78-108) from per-axis tables [sin | cos | cos2x]
+// evaluated on the host, in FP64, stored as f (FP64) or g = f - w*rho0.
+// ---------------------------------------------------------------------------
+template <class S>
+__global__ void __launch_bounds__(256)
+k_init(S *__restrict__ dst, const BoxDesc *__restrict__ boxes, const WorkTile *__restrict__ tiles,
+       int ntiles, int cpt, const double *__restrict__ tabs)
+{
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const WorkTile wt = tiles[t];
+        const BoxDesc &d = boxes[wt.box];
+        S *__restrict__ dp = dst + d.org + (long long)wt.z * d.sz;
+        for (int c = 0; c < cpt; ++c) {
+            const int q = c * (int)blockDim.x + (int)threadIdx.x;
+            if (q >= wt.n) break;
+            const int p = wt.p0 + q;
+            const int j = (int)(((unsigned long long)(unsigned)p * d.mag) >> 32);
+            const int i = p - j * d.nx;
+            const int X = d.lo[0] + i, Y = d.lo[1] + j, Z = d.lo[2] + wt.z;
+            const double *tx = tabs + d.tab[0], *ty = tabs + d.tab[1], *tz = tabs + d.tab[2];
+            const double sx = tx[X], cx = tx[d.gl[0] + X], c2x = tx[2 * d.gl[0] + X];
+            const double sy = ty[Y], cy = ty[d.gl[1] + Y], c2y = ty[2 * d.gl[1] + Y];
+            const double cz = tz[d.gl[2] + Z], c2z = tz[2 * d.gl[2] + Z];
+            const double u0 = d.u0, rho0 = d.rho0;
+            const double cs2 = 1.0 / 3.0;
+            const double u = u0 * sx * cy * cz;
+            const double v = -u0 * cx * sy * cz;
+            const double pr = rho0 * cs2 + rho0 * u0 * u0 / 16.0 * (c2x + c2y) * (c2z + 2.0);
+            double f[Q];
+            equilibrium64(pr, u, v, f);
+            const int o = i + j * d.sy;
+#pragma unroll
+            for (int ii = 0; ii < Q; ++ii) {
+                if constexpr (Store<S>::shifted) {
+                    const double g = f[ii] - weight(ii) * rho0;
+                    Store<S>::template st<false>(dp + (long long)ii * d.dstride + o, (float)g);
+                } else {
+                    Store<S>::template st<false>(dp + (long long)ii * d.dstride + o, f[ii]);
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Compensated FP64 reduction of {mass, px, py, pz, ke} (src/lbm.cpp:271-294)
+// over a range of work tiles.  Each thread keeps Neumaier sums; workgroups
+// merge them in a fixed tree, so the result does not depend on timing.
+// GATHER reads S_n = stream(P_{n-1}) through the pull offsets.
+// ---------------------------------------------------------------------------
+struct Acc { double s, c; };
+
+__device__ inline void acc_add(Acc &a, double v)
+{
+    const double t = a.s + v;
+    if (fabs(a.s) >= fabs(v)) a.c += (a.s - t) + v; else a.c += (v - t) + a.s;
+    a.s = t;
+}
+__device__ inline void acc_merge(Acc &a, const Acc &b)
+{
+    acc_add(a, b.s);
+    a.c += b.c;
+}
+
+constexpr int RED_BLOCK = 256;
+
+template <class S, bool WRAP, bool GATHER>
+__global__ void __launch_bounds__(RED_BLOCK)
+k_conserved(const S *__restrict__ src, const BoxDesc *__restrict__ boxes,
+            const WorkTile *__restrict__ tiles, int ntiles, int cpt, double *__restrict__ partial)
+{
+    using R = typename Store<S>::real;
+    Acc acc[5] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const WorkTile wt = tiles[t];
+        const BoxDesc &d = boxes[wt.box];
+        const S *__restrict__ sp = src + d.org + (long long)wt.z * d.sz;
+        for (int c = 0; c < cpt; ++c) {
+            const int q = c * (int)blockDim.x + (int)threadIdx.x;
+            if (q >= wt.n) break;
+            const int p = wt.p0 + q;
+            const int j = (int)(((unsigned long long)(unsigned)p * d.mag) >> 32);
+            const int i = p - j * d.nx;
+            const Nbr n = neighbours<WRAP>(d, i, j, wt.z);
+            R g[Q];
+            load_cell<S, GATHER, WRAP>(sp, d.dstride, i + j * d.sy, n, g,
+                                       std::make_integer_sequence<int, Q>{});
+            double f[Q];
+#pragma unroll
+            for (int ii = 0; ii < Q; ++ii) {
+                if constexpr (Store<S>::shifted) f[ii] = (double)g[ii] + weight(ii) * d.rho0;
+                else f[ii] = (double)g[ii];
+            }
+            const Moments64 m = moments64(f);
+            acc_add(acc[0], m.pl);
+            acc_add(acc[1], m.ul);
+            acc_add(acc[2], m.vl);
+            acc_add(acc[3], m.wl);
+            acc_add(acc[4], 0.5 * (m.ul * m.ul + m.vl * m.vl + m.wl * m.wl));
+        }
+    }
+    __shared__ double sh[2 * 5][RED_BLOCK];
+    for (int k = 0; k < 5; ++k) {
+        sh[2 * k][threadIdx.x] = acc[k].s;
+        sh[2 * k + 1][threadIdx.x] = acc[k].c;
+    }
+    __syncthreads();
+    for (int w = RED_BLOCK / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            for (int k = 0; k < 5; ++k) {
+                Acc a = {sh[2 * k][threadIdx.x], sh[2 * k + 1][threadIdx.x]};
+                const Acc b = {sh[2 * k][threadIdx.x + w], sh[2 * k + 1][threadIdx.x + w]};
+                acc_merge(a, b);
+                sh[2 * k][threadIdx.x] = a.s;
+                sh[2 * k + 1][threadIdx.x] = a.c;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 10) partial[blockIdx.x * 10 + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+// Second stage: one workgroup folds the per-workgroup partials, in order.
+__global__ void __launch_bounds__(RED_BLOCK)
+k_conserved_final(const double *__restrict__ partial, int nparts, double *__restrict__ out)
+{
+    Acc acc[5] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    for (int b = threadIdx.x; b < nparts; b += RED_BLOCK)
+        for (int k = 0; k < 5; ++k) {
+            const Acc v = {partial[b * 10 + 2 * k], partial[b * 10 + 2 * k + 1]};
+            acc_merge(acc[k], v);
+        }
+    __shared__ double sh[10][RED_BLOCK];
+    for (int k = 0; k < 5; ++k) {
+        sh[2 * k][threadIdx.x] = acc[k].s;
+        sh[2 * k + 1][threadIdx.x] = acc[k].c;
+    }
+    __syncthreads();
+    for (int w = RED_BLOCK / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            for (int k = 0; k < 5; ++k) {
+                Acc a = {sh[2 * k][threadIdx.x], sh[2 * k + 1][threadIdx.x]};
+                const Acc b = {sh[2 * k][threadIdx.x + w], sh[2 * k + 1][threadIdx.x + w]};
+                acc_merge(a, b);
+                sh[2 * k][threadIdx.x] = a.s;
+                sh[2 * k + 1][threadIdx.x] = a.c;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 5) out[threadIdx.x] = sh[2 * threadIdx.x][0] + sh[2 * threadIdx.x + 1][0];
+}
+
+// ---------------------------------------------------------------------------
+// Export / import of one box as doubles f in [ii][z][y][x] order (no ghosts).
+// Export with GATHER materialises S_n from the post-collision state.
+// ---------------------------------------------------------------------------
+template <class S, bool WRAP, bool GATHER>
+__global__ void __launch_bounds__(256)
+k_export(const S *__restrict__ src, const BoxDesc *__restrict__ boxes, int box,
+         double *__restrict__ out)
+{
+    using R = typename Store<S>::real;
+    const BoxDesc &d = boxes[box];
+    const long long ncell = (long long)d.nx * d.ny * d.nz;
+    for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < ncell;
+         c += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(c % d.nx);
+        const int j = (int)((c / d.nx) % d.ny);
+        const int k = (int)(c / ((long long)d.nx * d.ny));
+        const Nbr n = neighbours<WRAP>(d, i, j, k);
+        R g[Q];
+        load_cell<S, GATHER, WRAP>(src + d.org + (long long)k * d.sz, d.dstride, i + j * d.sy, n, g,
+                                   std::make_integer_sequence<int, Q>{});
+#pragma unroll
+        for (int ii = 0; ii < Q; ++ii) {
+            double f = (double)g[ii];
+            if constexpr (Store<S>::shifted) f += weight(ii) * d.rho0;
+            out[ii * ncell + c] = f;
+        }
+    }
+}
+
+template <class S>
+__global__ void __launch_bounds__(256)
+k_import(S *__restrict__ dst, const BoxDesc *__restrict__ boxes, int box,
+         const double *__restrict__ in)
+{
+    const BoxDesc &d = boxes[box];
+    const long long ncell = (long long)d.nx * d.ny * d.nz;
+    for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < ncell;
+         c += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(c % d.nx);
+        const int j = (int)((c / d.nx) % d.ny);
+        const int k = (int)(c / ((long long)d.nx * d.ny));
+        S *p = dst + d.org + (long long)k * d.sz + i + (long long)j * d.sy;
+        for (int ii = 0; ii < Q; ++ii) {
+            const double f = in[ii * ncell + c];
+            if constexpr (Store<S>::shifted)
+                Store<S>::template st<false>(p + ii * d.dstride, (float)(f - weight(ii) * d.rho0));
+            else
+                Store<S>::template st<false>(p + ii * d.dstride, f);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Halo pack / unpack over every segment of every tile in one launch.  Each
+// workgroup copies up to blockDim cells of one segment for all of its
+// directions.  Copies are bit-exact moves of the stored format.
+// ---------------------------------------------------------------------------
+template <class S>
+__global__ void __launch_bounds__(256)
+k_halo(const S *__restrict__ src, S *__restrict__ dst, const HaloSeg *__restrict__ segs,
+       const int2 *__restrict__ blocks, int nblocks)
+{
+    for (int b = blockIdx.x; b < nblocks; b += gridDim.x) {
+        const int2 bk = blocks[b];
+        const HaloSeg &sg = segs[bk.x];
+        const int c = bk.y + (int)threadIdx.x;
+        if (c >= sg.ncell) continue;
+        const int x = c % sg.ex;
+        const int y = (c / sg.ex) % sg.ey;
+        const int z = c / (sg.ex * sg.ey);
+        const long long so = sg.s_off + x + (long long)y * sg.s_sy + (long long)z * sg.s_sz;
+        const long long dofs = sg.d_off + x + (long long)y * sg.d_sy + (long long)z * sg.d_sz;
+        int slot = 0;
+#pragma unroll
+        for (int ii = 0; ii < Q; ++ii) {
+            if (sg.mask & (1u << ii)) {
+                const long long sdir = sg.pack ? ii : slot;
+                const long long ddir = sg.pack ? slot : ii;
+                dst[dofs + ddir * sg.d_ds] = src[so + sdir * sg.s_ds];
+                ++slot;
+            }
+        }
+    }
+}
+
+}  // namespace imex

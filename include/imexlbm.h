@@ -1,0 +1,155 @@
+/*
+ * imexlbm.h -- C ABI of the MI355X-native IMEXLBM hot path.
+ *
+ * The library owns every device buffer; the caller owns the context.  All
+ * calls on one context are stream-ordered on the context's own HIP stream and
+ * must come from one host thread.  Only imex_conserved, imex_get_f,
+ * imex_synchronize and imex_timed_steps block the host.  Every function
+ * returns IMEX_OK (0) or a negative status; imex_strerror names it and
+ * imex_last_error gives the detail of the most recent failure.
+ *
+ * Lattice state semantics follow the reference exactly: after n calls of
+ * Collision + Streaming the populations are the post-streaming state S_n
+ * (src/main.cpp:31-61, src/lbm.cpp:131-207).  Internally the library keeps a
+ * post-collision state and fuses the pull-stream of one step with the
+ * collision of the next; whatever reads S_n (imex_conserved, imex_get_f)
+ * gathers it on the fly, which is bit-identical because streaming is a copy.
+ */
+#ifndef IMEXLBM_H
+#define IMEXLBM_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IMEX_Q 27
+
+typedef struct imex_ctx imex_ctx;
+
+/* Storage precision of the populations.  IMEX_F64 stores f and computes in
+ * FP64 in the reference's operation order.  The others store the shifted
+ * populations g_i = f_i - w_i*rho0 and compute the collision in FP32. */
+enum imex_precision { IMEX_F64 = 0, IMEX_F32 = 1, IMEX_F16 = 2, IMEX_BF16 = 3 };
+
+/* Periodic ghost fill.  IMEX_WRAP: periodic index wrap inside the stream
+ * kernel, no ghost cells.  IMEX_TILED: the domain is split into subdomains
+ * with three ghost layers each, filled every step by one device pack and one
+ * device unpack launch over all 26 neighbour channels of all subdomains. */
+enum imex_mode { IMEX_WRAP = 0, IMEX_TILED = 1 };
+
+enum imex_status {
+    IMEX_OK = 0,
+    IMEX_EINVAL = -1,   /* bad argument                                  */
+    IMEX_ENOMEM = -2,   /* device or host allocation failed              */
+    IMEX_EHIP = -3,     /* a HIP runtime call failed                     */
+    IMEX_ENODEV = -4,   /* no usable GPU                                 */
+    IMEX_ESTATE = -5,   /* call out of order (e.g. before initialize)    */
+    IMEX_ERANGE = -6    /* size beyond what the kernels index            */
+};
+
+/* One Taylor-Green instance: the numbers the reference's System derives from
+ * input.in (src/System.cpp:22-28) and hands to LBM::LBM (src/lbm.hpp:179). */
+typedef struct imex_domain {
+    int nx, ny, nz;     /* global interior cells (glx, gly, glz)          */
+    double tau0;        /* relaxation parameter; rate is 1/(tau0+0.5)     */
+    double rho0;        /* reference density                              */
+    double u0;          /* Taylor-Green velocity amplitude                */
+} imex_domain;
+
+typedef struct imex_config {
+    int precision;        /* enum imex_precision                          */
+    int mode;             /* enum imex_mode                               */
+    int ntiles;           /* IMEX_TILED: subdomain count, factored as the
+                             reference's setup_Cartesian; 0 means 8       */
+    int halo_depth;       /* IMEX_TILED: ghost layers filled, 1..3; 0 = 1 */
+    int block;            /* threads per workgroup, multiple of 64; 0 = auto */
+    int cells_per_thread; /* 0 = auto                                     */
+    int persistent;       /* 0: one workgroup per work tile; >0: that many
+                             workgroups stride over the work-tile table   */
+    int nontemporal;      /* 1: streaming stores bypass the caches        */
+    int dir_pad;          /* extra elements between direction planes; -1 = auto */
+    int use_graph;        /* 1: imex_step replays a captured hipGraph     */
+    int sequential;       /* several domains: one launch per domain per
+                             step instead of one launch for all           */
+    int device;           /* HIP device ordinal                           */
+} imex_config;
+
+typedef struct imex_info {
+    long long cells;          /* interior cells over all domains          */
+    long long bytes_per_update; /* algorithmic bytes per lattice update:
+                                 2 * 27 * sizeof(storage)                  */
+    long long device_bytes;   /* bytes the context holds on the device    */
+    int storage_bytes;        /* sizeof one stored population             */
+    int ndomains;
+    int ntiles;               /* subdomains (IMEX_TILED) or 1 per domain  */
+    int work_tiles;           /* entries in the work-tile table           */
+    int block, cells_per_thread, persistent, nontemporal, dir_pad, use_graph;
+    int grid;                 /* tile grid rx, ry, rz as setup_Cartesian  */
+    int rx, ry, rz;
+} imex_info;
+
+/* Defaults for every field of imex_config (FP64, wrap, autotuned launch). */
+void imex_config_default(imex_config *cfg);
+
+/* LBM::LBM + the allocation half of LBM::Initialize (src/lbm.hpp:179-183,
+ * src/lbm.cpp:38-57).  ndomains > 1 creates a batch of independent periodic
+ * instances advanced by one launch per step (IMEX_WRAP only). */
+int imex_create(imex_ctx **out, const imex_config *cfg,
+                const imex_domain *domains, int ndomains);
+
+/* LBM::Initialize (src/lbm.cpp:38-111): analytic Taylor-Green field and the
+ * FP64 pressure-based equilibrium, computed on the device from per-axis
+ * sin/cos tables that the host evaluates with the C library. */
+int imex_initialize(imex_ctx *ctx);
+
+/* LBM::Collision (src/lbm.cpp:131-166).  Deferred: it is applied, fused with
+ * the next pull-stream, by imex_streaming or by whatever reads the state. */
+int imex_collision(imex_ctx *ctx);
+
+/* LBM::Streaming (src/lbm.cpp:168-207) including passf's ghost fill
+ * (src/mpi_view_transfer.cpp:451-456). */
+int imex_streaming(imex_ctx *ctx);
+
+/* nsteps x (LBM::Collision; LBM::Streaming), the driver's loop body
+ * (src/main.cpp:31-61).  Asynchronous. */
+int imex_step(imex_ctx *ctx, int nsteps);
+
+/* LBM::Conserved (src/lbm.cpp:266-306) for one domain: out = {mass, px, py,
+ * pz, ke} of S_n, summed in compensated FP64.  Blocks. */
+int imex_conserved(imex_ctx *ctx, int domain, double out[5]);
+
+/* Copy S_n of one domain to the host as doubles in the reference's
+ * (ii, i, j, k) order, host[((ii*nx + i)*ny + j)*nz + k].  Blocks. */
+int imex_get_f(imex_ctx *ctx, int domain, double *host);
+
+/* Replace one domain's state by S given in the same order (tests use it to
+ * drive collision and streaming from arbitrary inputs). */
+int imex_set_f(imex_ctx *ctx, int domain, const double *host);
+
+/* Wait for all work queued on the context. */
+int imex_synchronize(imex_ctx *ctx);
+
+/* Run nsteps steps between two synchronisations and report the wall time
+ * and the summed duration of the step kernels, measured with HIP events on
+ * the context's stream.  kernel_ms covers the fused step launches only. */
+int imex_timed_steps(imex_ctx *ctx, int nsteps, double *wall_ms,
+                     double *kernel_ms);
+
+/* Time candidate launch shapes (block, cells per thread, persistent grid,
+ * streaming stores) on the live arrays without changing the state, and keep
+ * the fastest.  Blocks.  best_ms (may be NULL) receives its step time. */
+int imex_autotune(imex_ctx *ctx, int reps, double *best_ms);
+
+int imex_query(imex_ctx *ctx, imex_info *info);
+void imex_destroy(imex_ctx *ctx);
+
+int imex_device_count(void);
+const char *imex_strerror(int status);
+const char *imex_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

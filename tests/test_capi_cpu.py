@@ -1,0 +1,75 @@
+"""CPU-side checks of the C ABI: the library loads, exports every entry point
+include/imexlbm.h declares, and fails cleanly where there is no GPU.  No
+compute call is made here."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+from tests.conftest import ROOT, gpu_available
+
+
+def _declared():
+    txt = open(os.path.join(ROOT, "include", "imexlbm.h")).read()
+    decl = r"^(?:int|void|const char)\s*\*?\s*(imex_[a-z_0-9]+)\s*\("
+    return sorted(set(re.findall(decl, txt, flags=re.M)))
+
+
+def test_header_and_binding_agree():
+    from imexlbm_amd import capi
+    assert sorted(capi.EXPORTS) == _declared()
+
+
+def test_library_exports_every_declared_symbol():
+    from imexlbm_amd import capi
+    lib = ctypes.CDLL(capi.LIB_PATH)
+    for name in _declared():
+        assert hasattr(lib, name), name
+
+
+def test_library_is_gfx950_code_object():
+    from imexlbm_amd import capi
+    out = subprocess.run(["/opt/rocm/bin/roc-obj-ls", capi.LIB_PATH], capture_output=True, text=True)
+    if out.returncode != 0:
+        pytest.skip("roc-obj-ls unavailable")
+    assert "gfx950" in out.stdout
+
+
+def test_strerror_and_defaults():
+    from imexlbm_amd import capi
+    L = capi.load()
+    assert L.imex_strerror(0) == b"ok"
+    assert L.imex_strerror(-2) == b"out of memory"
+    c = capi.default_config()
+    assert c.precision == capi.F64 and c.mode == capi.WRAP and c.ntiles == 8
+    assert c.halo_depth == 1 and c.dir_pad == -1
+
+
+@pytest.mark.skipif(gpu_available(), reason="checks the no-GPU error path")
+def test_create_without_gpu_fails_cleanly():
+    from imexlbm_amd import capi
+    with pytest.raises(capi.ImexError) as e:
+        capi.Context({"nx": 8, "ny": 8, "nz": 8, "tau0": 0.01, "rho0": 1.0, "u0": 0.05})
+    assert e.value.status in (-4, -3)
+
+
+def test_argument_validation_runs_before_device_use():
+    from imexlbm_amd import capi
+    dom = {"nx": 0, "ny": 8, "nz": 8, "tau0": 0.01, "rho0": 1.0, "u0": 0.05}
+    with pytest.raises(capi.ImexError) as e:
+        capi.Context(dom)
+    assert e.value.status == -1
+    dom = {"nx": 8, "ny": 8, "nz": 8, "tau0": 0.01, "rho0": 1.0, "u0": 0.05}
+    with pytest.raises(capi.ImexError):
+        capi.Context(dom, halo_depth=4, mode=capi.TILED)
+    with pytest.raises(capi.ImexError):
+        capi.Context([dom, dom], mode=capi.TILED)
+
+
+def test_driver_aborts_without_deck(tmp_path):
+    exe = os.path.join(ROOT, "imexlbm_amd", "bin", "imexlbm")
+    r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True)
+    assert r.returncode != 0
+    assert "cannot open input.in" in r.stderr
