@@ -14,6 +14,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -951,6 +952,9 @@ int imex_autotune(imex_ctx *c, int reps, double *best_ms)
         float ms = 0.f;
         HIPC(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
         const double t = ms / reps;
+        if (std::getenv("IMEXLBM_AUTOTUNE_LOG"))
+            std::fprintf(stderr, "autotune block=%d cpt=%d persistent=%d nt=%d: %.4f ms\n",
+                         k.block, k.cpt, k.pers, k.nt, t);
         if (t < best_t) { best_t = t; best = k; }
     }
     c->block = best.block; c->cpt = best.cpt; c->persistent = best.pers; c->nt = best.nt;

@@ -109,7 +109,7 @@ __device__ inline void store_cell(S *__restrict__ base, long long ds, int o,
 template <class S, bool WRAP, int OP, bool NT>
 __global__ void __launch_bounds__(512)
 k_step(const S *__restrict__ src, S *__restrict__ dst, const BoxDesc *__restrict__ boxes,
-       const WorkTile *__restrict__ tiles, int ntiles, int cpt)
+       const WorkTile *__restrict__ tiles, int ntiles, int /*cells_per_thread: sets the tile size only*/)
 {
     using R = typename Store<S>::real;
     constexpr bool GATHER = (OP != OP_COLLIDE);
@@ -119,9 +119,7 @@ k_step(const S *__restrict__ src, S *__restrict__ dst, const BoxDesc *__restrict
         const long long plane = d.org + (long long)wt.z * d.sz;
         const S *__restrict__ sp = src + plane;
         S *__restrict__ dp = dst + plane;
-        for (int c = 0; c < cpt; ++c) {
-            const int q = c * (int)blockDim.x + (int)threadIdx.x;
-            if (q >= wt.n) break;
+        for (int q = (int)threadIdx.x; q < wt.n; q += (int)blockDim.x) {
             const int p = wt.p0 + q;
             const int j = (int)(((unsigned long long)(unsigned)p * d.mag) >> 32);
             const int i = p - j * d.nx;
@@ -143,17 +141,15 @@ k_step(const S *__restrict__ src, S *__restrict__ dst, const BoxDesc *__restrict
 // evaluated on the host, in FP64, stored as f (FP64) or g = f - w*rho0.
 // ---------------------------------------------------------------------------
 template <class S>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(512)
 k_init(S *__restrict__ dst, const BoxDesc *__restrict__ boxes, const WorkTile *__restrict__ tiles,
-       int ntiles, int cpt, const double *__restrict__ tabs)
+       int ntiles, int /*cells_per_thread: sets the tile size only*/, const double *__restrict__ tabs)
 {
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const WorkTile wt = tiles[t];
         const BoxDesc &d = boxes[wt.box];
         S *__restrict__ dp = dst + d.org + (long long)wt.z * d.sz;
-        for (int c = 0; c < cpt; ++c) {
-            const int q = c * (int)blockDim.x + (int)threadIdx.x;
-            if (q >= wt.n) break;
+        for (int q = (int)threadIdx.x; q < wt.n; q += (int)blockDim.x) {
             const int p = wt.p0 + q;
             const int j = (int)(((unsigned long long)(unsigned)p * d.mag) >> 32);
             const int i = p - j * d.nx;
@@ -208,7 +204,7 @@ constexpr int RED_BLOCK = 256;
 template <class S, bool WRAP, bool GATHER>
 __global__ void __launch_bounds__(RED_BLOCK)
 k_conserved(const S *__restrict__ src, const BoxDesc *__restrict__ boxes,
-            const WorkTile *__restrict__ tiles, int ntiles, int cpt, double *__restrict__ partial)
+            const WorkTile *__restrict__ tiles, int ntiles, int /*cells_per_thread: sets the tile size only*/, double *__restrict__ partial)
 {
     using R = typename Store<S>::real;
     Acc acc[5] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}};
@@ -216,9 +212,7 @@ k_conserved(const S *__restrict__ src, const BoxDesc *__restrict__ boxes,
         const WorkTile wt = tiles[t];
         const BoxDesc &d = boxes[wt.box];
         const S *__restrict__ sp = src + d.org + (long long)wt.z * d.sz;
-        for (int c = 0; c < cpt; ++c) {
-            const int q = c * (int)blockDim.x + (int)threadIdx.x;
-            if (q >= wt.n) break;
+        for (int q = (int)threadIdx.x; q < wt.n; q += (int)blockDim.x) {
             const int p = wt.p0 + q;
             const int j = (int)(((unsigned long long)(unsigned)p * d.mag) >> 32);
             const int i = p - j * d.nx;
