@@ -150,6 +150,8 @@ def main():
               file=sys.stderr, flush=True)
 
     h = results[args.headline]
+    print("bench: copy probe", file=sys.stderr, flush=True)
+    copy_gbs = capi.copy_probe(8 << 30, 10)
     cpu = None
     if not args.no_cpu_baseline:
         print("bench: cpu baseline", file=sys.stderr, flush=True)
@@ -184,6 +186,8 @@ def main():
             "unit": "GB/s",
             "frac": h["roofline_frac"],
             "traffic": traffic_from_profiles(args.headline, args.n),
+            "copy_ceiling": round(copy_gbs, 1),
+            "frac_of_copy": round(h["achieved_gbs"] / copy_gbs, 4),
         },
         "cpu_baseline": cpu,
     }

@@ -24,6 +24,7 @@ EXPORTS = (
     "imex_streaming", "imex_step", "imex_conserved", "imex_get_f", "imex_set_f",
     "imex_synchronize", "imex_timed_steps", "imex_autotune", "imex_query",
     "imex_destroy", "imex_device_count", "imex_strerror", "imex_last_error",
+    "imex_copy_probe",
 )
 
 
@@ -79,6 +80,7 @@ def load():
         L.imex_destroy.argtypes = [vp]
         L.imex_destroy.restype = None
         L.imex_device_count.restype = i
+        L.imex_copy_probe.argtypes = [i, ctypes.c_longlong, i, dp]
         L.imex_strerror.restype = ctypes.c_char_p
         L.imex_strerror.argtypes = [i]
         L.imex_last_error.restype = ctypes.c_char_p
@@ -88,6 +90,13 @@ def load():
 
 def device_count() -> int:
     return int(load().imex_device_count())
+
+
+def copy_probe(nbytes=8 << 30, reps=10, device=0) -> float:
+    """Read+write GB/s of a plain device copy of nbytes."""
+    out = np.zeros(1, np.float64)
+    _check(load().imex_copy_probe(int(device), int(nbytes), int(reps), _dptr(out)))
+    return float(out[0])
 
 
 def _check(status):

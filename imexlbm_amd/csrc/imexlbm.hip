@@ -965,6 +965,46 @@ int imex_autotune(imex_ctx *c, int reps, double *best_ms)
     return IMEX_OK;
 }
 
+int imex_copy_probe(int device, long long bytes, int reps, double *gbs)
+{
+    if (bytes < (1 << 20) || reps < 1 || !gbs) return fail(IMEX_EINVAL, "copy probe needs >= 1 MiB, reps >= 1");
+    HIPC(hipSetDevice(device));
+    const long long n = bytes / 16;
+    void *a = nullptr, *b = nullptr;
+    if (hipMalloc(&a, n * 16) != hipSuccess) return fail(IMEX_ENOMEM, "copy probe allocation failed");
+    if (hipMalloc(&b, n * 16) != hipSuccess) {
+        (void)hipFree(a);
+        return fail(IMEX_ENOMEM, "copy probe allocation failed");
+    }
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int r = IMEX_OK;
+    int ncu = 256;
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device);
+    const int grid = (int)std::min<long long>((n + 255) / 256, (long long)ncu * 64);
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess)
+        r = fail(IMEX_EHIP, "copy probe stream/event creation failed");
+    if (r == IMEX_OK) {
+        hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, s, (const u32x4 *)a, (u32x4 *)b, n);
+        (void)hipEventRecord(e0, s);
+        for (int k = 0; k < reps; ++k)
+            hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, s, (const u32x4 *)a, (u32x4 *)b, n);
+        (void)hipEventRecord(e1, s);
+        float ms = 0.f;
+        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+            r = fail(IMEX_EHIP, "copy probe timing failed");
+        else
+            *gbs = 2.0 * (double)n * 16.0 * reps / (ms * 1e-3) / 1e9;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (s) (void)hipStreamDestroy(s);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return r;
+}
+
 int imex_query(imex_ctx *c, imex_info *info)
 {
     if (!c || !info) return fail(IMEX_EINVAL, "null argument");

@@ -143,6 +143,13 @@ int imex_timed_steps(imex_ctx *ctx, int nsteps, double *wall_ms,
 int imex_autotune(imex_ctx *ctx, int reps, double *best_ms);
 
 int imex_query(imex_ctx *ctx, imex_info *info);
+
+/* Measured HBM ceiling: a plain device-to-device copy of `bytes` (16 bytes
+ * per lane, streaming stores), timed over `reps` launches; *gbs receives read
+ * plus write bytes per second / 1e9.  No counterpart in the reference, whose
+ * phase profiler assumes a fixed 1600 GB/s peak instead
+ * (src/profile.hpp:67-73); the step kernels are reported against this. */
+int imex_copy_probe(int device, long long bytes, int reps, double *gbs);
 void imex_destroy(imex_ctx *ctx);
 
 int imex_device_count(void);
