@@ -68,7 +68,7 @@ def run_mode(capi, prec, n, re, u0, steps, warmup, autotune, use_graph):
         "roofline_frac": round(achieved / HBM_PEAK_GBS, 4),
         "roofline_mlups": round(HBM_PEAK_GBS * 1e9 / bpu / 1e6, 1),
         "launch": {k: info[k] for k in ("block", "cells_per_thread", "persistent", "nontemporal",
-                                        "dir_pad", "use_graph")},
+                                        "dir_pad", "use_graph", "xcd_groups")},
         "autotune_best_ms": None if tuned_ms is None else round(tuned_ms, 4),
         "mass": cons[0], "ke": cons[4],
     }

@@ -36,7 +36,8 @@ class Domain(ctypes.Structure):
 class Config(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int) for n in (
         "precision", "mode", "ntiles", "halo_depth", "block", "cells_per_thread",
-        "persistent", "nontemporal", "dir_pad", "use_graph", "sequential", "device")]
+        "persistent", "nontemporal", "dir_pad", "use_graph", "sequential", "device",
+        "xcd_groups", "contiguous")]
 
 
 class Info(ctypes.Structure):
@@ -44,7 +45,7 @@ class Info(ctypes.Structure):
                 ("device_bytes", ctypes.c_longlong)] + [(n, ctypes.c_int) for n in (
                     "storage_bytes", "ndomains", "ntiles", "work_tiles", "block",
                     "cells_per_thread", "persistent", "nontemporal", "dir_pad", "use_graph",
-                    "grid", "rx", "ry", "rz")]
+                    "grid", "rx", "ry", "rz", "xcd_groups", "contiguous")]
 
 
 class ImexError(RuntimeError):

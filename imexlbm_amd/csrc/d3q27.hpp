@@ -71,8 +71,9 @@ template <> struct Store<_Float16> {
     }
 };
 
-// bfloat16 as raw bits: widening is a shift, narrowing rounds to nearest
-// even (NaN kept quiet).
+// bfloat16 as raw bits: widening is a shift.  Device code narrows with the
+// hardware conversion (round to nearest even); the host form below is the same
+// rounding, for host-side use.
 __host__ __device__ inline float bf16_to_float(uint16_t b)
 {
     union { uint32_t u; float f; } c;
@@ -98,7 +99,7 @@ template <> struct Store<bf16_t> {
     }
     template <bool NT> __device__ static void st(bf16_t *p, float v)
     {
-        const uint16_t b = float_to_bf16(v);
+        const uint16_t b = __builtin_bit_cast(uint16_t, (__bf16)v);   // round to nearest even
         uint16_t *q = reinterpret_cast<uint16_t *>(p);
         if constexpr (NT) __builtin_nontemporal_store(b, q); else *q = b;
     }

@@ -138,7 +138,8 @@ struct imex_ctx {
     bool pending = false;      // a Collision() not yet applied
     bool ghosts_valid = false; // tiled: ghost layers of pool[cur] are current
 
-    int block = 256, cpt = 1, persistent = 0, nt = 0, dir_pad = 0, halo_depth = 1;
+    int block = 256, cpt = 1, persistent = 0, nt = 0, dir_pad = 0, halo_depth = 1, xcds = 8;
+    int contiguous = 0;
     hipGraph_t graph[2] = {nullptr, nullptr};
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
 
@@ -185,31 +186,40 @@ int layout_boxes(imex_ctx *c)
                     return fail(IMEX_EINVAL, "tile thinner than the halo depth");
             }
             b.nx = n[0]; b.ny = n[1]; b.nz = n[2];
-            long long vol, org;
+            // f[z][ii][y][x]: 27 direction sub-planes per z-plane
+            long long rows, zplanes, org_xy, zorg;
             if (c->tiled) {
                 const long long xoff = std::max<long long>(align, 3);
                 b.sy = (int)round_up(xoff + n[0] + 3, align);
-                b.sz = b.sy * (n[1] + 6);
-                vol = (long long)b.sz * (n[2] + 6);
-                org = xoff + 3LL * b.sy + 3LL * b.sz;
+                rows = n[1] + 6;
+                zplanes = n[2] + 6;
+                org_xy = xoff + 3LL * b.sy;
             } else {
                 b.sy = n[0];
-                b.sz = n[0] * n[1];
-                vol = (long long)b.sz * n[2];
-                org = 0;
+                rows = n[1];
+                zplanes = n[2];
+                org_xy = 0;
             }
-            if ((long long)b.sz * (n[2] + 6) >= (1LL << 31) ||
-                (long long)n[0] * n[1] * (long long)n[0] >= (1LL << 32))
-                return fail(IMEX_ERANGE, "box too large for 32-bit in-plane indexing");
-            b.dstride = round_up(vol, align) + std::max(c->dir_pad, 0);
-            b.org = start + org;
+            b.dstride = round_up((long long)b.sy * rows, align) + std::max(c->dir_pad, 0);
+            const long long sz = 27LL * b.dstride;
+            // one z-plane (all directions) must fit a 2 GiB buffer window, and
+            // in-plane cell offsets must fit 32-bit indexing
+            if (sz * c->ssize >= (1LL << 31) || (long long)n[0] * n[1] * (long long)n[0] >= (1LL << 32))
+                return fail(IMEX_ERANGE, "z-plane too large for 32-bit buffer addressing");
+            b.sz = (int)sz;
+            zorg = c->tiled ? 3 * sz : 0;
+            const long long vol = sz * zplanes;
+            b.base = start;
+            b.zorg = zorg;
+            b.org_xy = (int)org_xy;
+            b.org = start + zorg + org_xy;
             b.mag = ((1ULL << 32) + (unsigned long long)n[0] - 1) / (unsigned long long)n[0];
             b.inst = (int)in;
             b.omega = I.omega;
             b.omega_f = (float)I.omega;
             b.rho0 = I.dom.rho0;
             b.u0 = I.dom.u0;
-            start += round_up(27LL * b.dstride, align * 2);
+            start += round_up(vol, align * 2);
             c->boxes.push_back(b);
             c->box_coord.insert(c->box_coord.end(), p, p + 3);
             c->cells += (long long)n[0] * n[1] * n[2];
@@ -349,7 +359,7 @@ int launch_step_t(imex_ctx *c, int op, int src, int dst, int t0, int n)
     const int grid = c->persistent > 0 ? std::min(c->persistent, n) : n;
     const dim3 g(grid), b(c->block);
     const bool nt = c->nt != 0;
-#define LAUNCH(W, O, N) hipLaunchKernelGGL((k_step<S, W, O, N>), g, b, 0, c->stream, sp, dp, c->d_boxes, tl, n, c->cpt)
+#define LAUNCH(W, O, N) hipLaunchKernelGGL((k_step<S, W, O, N>), g, b, 0, c->stream, sp, dp, c->d_boxes, tl, n, c->xcds)
     if (!c->tiled) {
         if (op == OP_FUSED) { if (nt) LAUNCH(true, OP_FUSED, true); else LAUNCH(true, OP_FUSED, false); }
         else if (op == OP_COLLIDE) LAUNCH(true, OP_COLLIDE, false);
@@ -609,6 +619,7 @@ void imex_config_default(imex_config *cfg)
     cfg->ntiles = 8;
     cfg->halo_depth = 1;
     cfg->dir_pad = -1;
+    cfg->nontemporal = -1;
 }
 
 const char *imex_strerror(int s)
@@ -686,8 +697,9 @@ int imex_create(imex_ctx **out, const imex_config *cfg_in, const imex_domain *do
     c->block = cfg.block ? cfg.block : 256;
     c->cpt = cfg.cells_per_thread ? cfg.cells_per_thread : 1;
     c->persistent = std::max(cfg.persistent, 0);
-    c->nt = cfg.nontemporal ? 1 : 0;
+    c->nt = cfg.nontemporal >= 0 ? (cfg.nontemporal ? 1 : 0) : 1;
     c->dir_pad = cfg.dir_pad >= 0 ? cfg.dir_pad : 0;
+    c->xcds = cfg.xcd_groups > 0 ? cfg.xcd_groups : 8;
     auto bail = [&](int r) { imex_destroy(c); return r; };
 
     if (hipSetDevice(c->device) != hipSuccess) return bail(fail(IMEX_EHIP, "hipSetDevice failed"));
@@ -716,9 +728,17 @@ int imex_create(imex_ctx **out, const imex_config *cfg_in, const imex_domain *do
     if (c->tiled) build_halo(c);
 
     const size_t pool_bytes = (size_t)c->pool_elems * c->ssize;
-    for (int k = 0; k < 2; ++k)
-        if (hipMalloc(&c->pool[k], pool_bytes) != hipSuccess)
+    c->contiguous = cfg.contiguous ? 1 : 0;
+    for (int k = 0; k < 2; ++k) {
+        if (c->contiguous &&
+            hipExtMallocWithFlags(&c->pool[k], pool_bytes, hipDeviceMallocContiguous) != hipSuccess) {
+            (void)hipGetLastError();
+            c->pool[k] = nullptr;
+            c->contiguous = 0;
+        }
+        if (!c->pool[k] && hipMalloc(&c->pool[k], pool_bytes) != hipSuccess)
             return bail(fail(IMEX_ENOMEM, "population pool allocation failed (" + std::to_string(pool_bytes) + " bytes)"));
+    }
     c->device_bytes = 2LL * (long long)pool_bytes;
     if (c->tiled) {
         const size_t xb = std::max<long long>(c->xbuf_elems, 1) * (size_t)c->ssize;
@@ -928,18 +948,19 @@ int imex_autotune(imex_ctx *c, int reps, double *best_ms)
     TRY(flush_pending(c));
     TRY(ensure_ghosts(c));
     HIPC(hipStreamSynchronize(c->stream));
-    struct Cand { int block, cpt, pers, nt; };
+    struct Cand { int block, cpt, pers, nt, xcds; };
     std::vector<Cand> cands;
-    const int pers_opts[3] = {0, c->ncu * 4, c->ncu * 8};
+    const int pers_opts[2] = {0, c->ncu * 8};
     for (int blk : {128, 256, 512})
         for (int cpt : {1, 2, 4})
             for (int pers : pers_opts)
-                for (int nt : {0, 1}) cands.push_back({blk, cpt, pers, nt});
-    const Cand orig = {c->block, c->cpt, c->persistent, c->nt};
+                for (int nt : {0, 1})
+                    for (int x : {1, 8}) cands.push_back({blk, cpt, pers, nt, x});
+    const Cand orig = {c->block, c->cpt, c->persistent, c->nt, c->xcds};
     Cand best = orig;
     double best_t = 1e300;
     for (const Cand &k : cands) {
-        c->block = k.block; c->cpt = k.cpt; c->persistent = k.pers; c->nt = k.nt;
+        c->block = k.block; c->cpt = k.cpt; c->persistent = k.pers; c->nt = k.nt; c->xcds = k.xcds;
         build_tiles(c);
         TRY(upload_tiles(c));
         // warm-up, then timed: fused launches cur -> other without swapping,
@@ -953,11 +974,12 @@ int imex_autotune(imex_ctx *c, int reps, double *best_ms)
         HIPC(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
         const double t = ms / reps;
         if (std::getenv("IMEXLBM_AUTOTUNE_LOG"))
-            std::fprintf(stderr, "autotune block=%d cpt=%d persistent=%d nt=%d: %.4f ms\n",
-                         k.block, k.cpt, k.pers, k.nt, t);
+            std::fprintf(stderr, "autotune block=%d cpt=%d persistent=%d nt=%d xcds=%d: %.4f ms\n",
+                         k.block, k.cpt, k.pers, k.nt, k.xcds, t);
         if (t < best_t) { best_t = t; best = k; }
     }
     c->block = best.block; c->cpt = best.cpt; c->persistent = best.pers; c->nt = best.nt;
+    c->xcds = best.xcds;
     build_tiles(c);
     TRY(upload_tiles(c));
     drop_graphs(c);
@@ -979,24 +1001,28 @@ int imex_copy_probe(int device, long long bytes, int reps, double *gbs)
     hipStream_t s = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int r = IMEX_OK;
-    int ncu = 256;
-    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device);
-    const int grid = (int)std::min<long long>((n + 255) / 256, (long long)ncu * 64);
+    const int grid = (int)((n + 1023) / 1024);
     if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
         hipEventCreate(&e1) != hipSuccess)
         r = fail(IMEX_EHIP, "copy probe stream/event creation failed");
-    if (r == IMEX_OK) {
-        hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, s, (const u32x4 *)a, (u32x4 *)b, n);
+    // best of plain and streaming stores
+    double best = 0.0;
+    for (int nt = 0; nt < 2 && r == IMEX_OK; ++nt) {
+        auto launch = [&]() {
+            if (nt) hipLaunchKernelGGL(k_copy<true>, dim3(grid), dim3(256), 0, s, (const u32x4 *)a, (u32x4 *)b, n);
+            else hipLaunchKernelGGL(k_copy<false>, dim3(grid), dim3(256), 0, s, (const u32x4 *)a, (u32x4 *)b, n);
+        };
+        launch();
         (void)hipEventRecord(e0, s);
-        for (int k = 0; k < reps; ++k)
-            hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, s, (const u32x4 *)a, (u32x4 *)b, n);
+        for (int k = 0; k < reps; ++k) launch();
         (void)hipEventRecord(e1, s);
         float ms = 0.f;
         if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
             r = fail(IMEX_EHIP, "copy probe timing failed");
         else
-            *gbs = 2.0 * (double)n * 16.0 * reps / (ms * 1e-3) / 1e9;
+            best = std::max(best, 2.0 * (double)n * 16.0 * reps / (ms * 1e-3) / 1e9);
     }
+    if (r == IMEX_OK) *gbs = best;
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     if (s) (void)hipStreamDestroy(s);
@@ -1022,6 +1048,8 @@ int imex_query(imex_ctx *c, imex_info *info)
     info->nontemporal = c->nt;
     info->dir_pad = c->dir_pad;
     info->use_graph = c->cfg.use_graph;
+    info->xcd_groups = c->xcds;
+    info->contiguous = c->contiguous;
     info->grid = c->tiled ? 1 : 0;
     info->rx = c->rgrid[0]; info->ry = c->rgrid[1]; info->rz = c->rgrid[2];
     return IMEX_OK;

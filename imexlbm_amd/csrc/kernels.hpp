@@ -1,11 +1,14 @@
 // kernels.hpp -- gfx950 kernels of the IMEXLBM hot path.
 //
-// Data layout in HBM: structure of arrays, one plane per direction,
-// f[ii][z][y][x] with x stride-1.  A "box" is one stored lattice block: a
-// whole periodic domain (no ghost cells, periodic wrap in the index math) or
-// one tile of a tiled domain (three ghost layers, rows padded so interior
-// rows start on a 128-byte boundary).  Two pools A and B with identical
-// layout hold the ping-pong state.
+// Data layout in HBM: structure of arrays with x stride-1, grouped by
+// z-plane: f[z][ii][y][x].  Each z-plane holds the 27 direction sub-planes
+// back to back (dstride elements apart, sz = 27*dstride between z-planes), so
+// everything one z-plane of cells touches lies in one window that a single
+// buffer descriptor can address with 32-bit offsets.  A "box" is one stored
+// lattice block: a whole periodic domain (no ghost cells, periodic wrap in the
+// index math) or one tile of a tiled domain (three ghost layers, rows padded
+// so interior rows start on a 128-byte boundary).  Two pools A and B with
+// identical layout hold the ping-pong state.
 //
 // Work is described by a table of work tiles, each a run of up to
 // block*cells_per_thread consecutive cells of one z-plane of one box.  Every
@@ -33,7 +36,9 @@ struct BoxDesc {
     double omega;       // 1/(tau0+0.5)
     double rho0, u0;
     float omega_f;
-    int pad_;
+    int org_xy;         // in-plane offset of interior cell (0,0) from the row-0 start
+    long long base;     // element offset of the box allocation (ghosts included)
+    long long zorg;     // offset of interior z-plane 0 from base (3*sz with ghosts)
 };
 
 struct WorkTile { int box, z, p0, n; };
@@ -54,7 +59,7 @@ struct HaloSeg {
 enum { OP_COLLIDE = 0, OP_FUSED = 1, OP_STREAM = 2 };
 
 // Offsets, relative to a cell, of the cells at x-1, x+1, y-1, y+1, z-1, z+1.
-struct Nbr { int xm, xp, ym, yp, zm, zp; };
+struct Nbr { int xm, xp, ym, yp; long long zm, zp; };
 
 template <bool WRAP>
 __device__ inline Nbr neighbours(const BoxDesc &d, int i, int j, int k)
@@ -65,8 +70,8 @@ __device__ inline Nbr neighbours(const BoxDesc &d, int i, int j, int k)
         n.xp = (i == d.nx - 1) ? 1 - d.nx : 1;
         n.ym = (j == 0) ? (d.ny - 1) * d.sy : -d.sy;
         n.yp = (j == d.ny - 1) ? (1 - d.ny) * d.sy : d.sy;
-        n.zm = (k == 0) ? (d.nz - 1) * d.sz : -d.sz;
-        n.zp = (k == d.nz - 1) ? (1 - d.nz) * d.sz : d.sz;
+        n.zm = (k == 0) ? (long long)(d.nz - 1) * d.sz : -(long long)d.sz;
+        n.zp = (k == d.nz - 1) ? (long long)(1 - d.nz) * d.sz : (long long)d.sz;
     } else {
         n.xm = -1; n.xp = 1;
         n.ym = -d.sy; n.yp = d.sy;
@@ -77,9 +82,9 @@ __device__ inline Nbr neighbours(const BoxDesc &d, int i, int j, int k)
 
 // Pull source of direction ii: the cell at x - e_ii.
 template <int ii>
-__device__ inline int pull_delta(const Nbr &n)
+__device__ inline long long pull_delta(const Nbr &n)
 {
-    int o = 0;
+    long long o = 0;
     if constexpr (EX[ii] == 1) o += n.xm; else if constexpr (EX[ii] == -1) o += n.xp;
     if constexpr (EY[ii] == 1) o += n.ym; else if constexpr (EY[ii] == -1) o += n.yp;
     if constexpr (EZ[ii] == 1) o += n.zm; else if constexpr (EZ[ii] == -1) o += n.zp;
@@ -102,36 +107,149 @@ __device__ inline void store_cell(S *__restrict__ base, long long ds, int o,
     (Store<S>::template st<NT>(base + (long long)I * ds + o, f[I]), ...);
 }
 
+// Buffer-descriptor access used by the step kernel.  A descriptor's base is
+// the start of one z-plane of a box; the per-lane offset is the in-plane cell
+// (x, y) in bytes and the scalar offset selects the direction sub-plane, so a
+// load or store needs no per-lane 64-bit address arithmetic.
+typedef __amdgpu_buffer_rsrc_t rsrc_t;
+
+__device__ inline rsrc_t plane_rsrc(const void *p)
+{
+    // stride 0 (raw), 2 GiB window, gfx9 default dword3
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, 0x7fffffff, 0x00020000);
+}
+
+template <class S> struct BufIO;
+
+template <> struct BufIO<double> {
+    __device__ static double ld(rsrc_t r, int vo, int so)
+    {
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0));
+    }
+    template <bool NT> __device__ static void st(double v, rsrc_t r, int vo, int so)
+    {
+        using B = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(B, v), r, vo, so, NT ? 2 : 0);
+    }
+};
+
+template <> struct BufIO<float> {
+    __device__ static float ld(rsrc_t r, int vo, int so)
+    {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0));
+    }
+    template <bool NT> __device__ static void st(float v, rsrc_t r, int vo, int so)
+    {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, vo, so, NT ? 2 : 0);
+    }
+};
+
+template <> struct BufIO<_Float16> {
+    __device__ static float ld(rsrc_t r, int vo, int so)
+    {
+        return (float)__builtin_bit_cast(_Float16, __builtin_amdgcn_raw_buffer_load_b16(r, vo, so, 0));
+    }
+    template <bool NT> __device__ static void st(float v, rsrc_t r, int vo, int so)
+    {
+        const _Float16 h = (_Float16)v;
+        __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, h), r, vo, so, NT ? 2 : 0);
+    }
+};
+
+template <> struct BufIO<bf16_t> {
+    __device__ static float ld(rsrc_t r, int vo, int so)
+    {
+        return bf16_to_float(__builtin_amdgcn_raw_buffer_load_b16(r, vo, so, 0));
+    }
+    template <bool NT> __device__ static void st(float v, rsrc_t r, int vo, int so)
+    {
+        __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, (__bf16)v), r, vo, so,
+                                              NT ? 2 : 0);
+    }
+};
+
+// In-plane pull offset (elements) of direction ii: the cell at (x, y) - e.
+template <int ii>
+__device__ inline int pull_xy(const Nbr &n)
+{
+    int o = 0;
+    if constexpr (EX[ii] == 1) o += n.xm; else if constexpr (EX[ii] == -1) o += n.xp;
+    if constexpr (EY[ii] == 1) o += n.ym; else if constexpr (EY[ii] == -1) o += n.yp;
+    return o;
+}
+
+template <class S, bool GATHER, int... I>
+__device__ inline void load_cell_buf(rsrc_t rm, rsrc_t r0, rsrc_t rp, int dsb, int xy, const Nbr &n,
+                                     typename Store<S>::real (&f)[Q], std::integer_sequence<int, I...>)
+{
+    // the source of direction ii lies in plane z - e_z
+    ((f[I] = BufIO<S>::ld((!GATHER || EZ[I] == 0) ? r0 : (EZ[I] == 1 ? rm : rp),
+                          (xy + (GATHER ? pull_xy<I>(n) : 0)) * (int)sizeof(S), I * dsb)), ...);
+}
+
+template <class S, bool NT, int... I>
+__device__ inline void store_cell_buf(rsrc_t r, int dsb, int xy, const typename Store<S>::real (&f)[Q],
+                                      std::integer_sequence<int, I...>)
+{
+    (BufIO<S>::template st<NT>(f[I], r, xy * (int)sizeof(S), I * dsb), ...);
+}
+
 // ---------------------------------------------------------------------------
 // Step kernel.  OP_COLLIDE: collide in place (src == dst).  OP_FUSED: pull
 // from src, collide, store to dst.  OP_STREAM: pull only (a pure copy).
 // ---------------------------------------------------------------------------
+// Workgroups are dealt to the XCDs round-robin (workgroup w runs on XCD
+// w % X), and each XCD has its own L2.  xcd_slot renumbers workgroups so that
+// the ones sharing an XCD take one contiguous range of work tiles: the cache
+// line an x-shifted direction straddles at a tile boundary is then read from
+// the same L2 by both tiles instead of being fetched from HBM twice.
+__device__ inline int xcd_slot(int w, int g, int x)
+{
+    if (x <= 1) return w;
+    const int q = g / x, rem = g % x, k = w % x;
+    return k * q + min(k, rem) + w / x;
+}
+
 template <class S, bool WRAP, int OP, bool NT>
 __global__ void __launch_bounds__(512)
 k_step(const S *__restrict__ src, S *__restrict__ dst, const BoxDesc *__restrict__ boxes,
-       const WorkTile *__restrict__ tiles, int ntiles, int /*cells_per_thread: sets the tile size only*/)
+       const WorkTile *__restrict__ tiles, int ntiles, int xcds)
 {
     using R = typename Store<S>::real;
     constexpr bool GATHER = (OP != OP_COLLIDE);
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const int r = xcd_slot((int)blockIdx.x, (int)gridDim.x, xcds);
+    // one tile per workgroup, or (persistent grid) a contiguous run of tiles
+    const int per = ((int)gridDim.x == ntiles) ? 1 : (ntiles + (int)gridDim.x - 1) / (int)gridDim.x;
+    const int t_end = min(ntiles, (r + 1) * per);
+#pragma clang loop unroll(disable)
+    for (int t = r * per; t < t_end; ++t) {
         const WorkTile wt = tiles[t];
         const BoxDesc &d = boxes[wt.box];
-        const long long plane = d.org + (long long)wt.z * d.sz;
-        const S *__restrict__ sp = src + plane;
-        S *__restrict__ dp = dst + plane;
+        const int z = wt.z;
+        int zm = z - 1, zp = z + 1;
+        if constexpr (WRAP) {
+            if (zm < 0) zm = d.nz - 1;
+            if (zp == d.nz) zp = 0;
+        }
+        const S *sbox = src + d.base + d.zorg;
+        const rsrc_t r0 = plane_rsrc(sbox + (long long)z * d.sz);
+        const rsrc_t rm = plane_rsrc(sbox + (long long)zm * d.sz);
+        const rsrc_t rp = plane_rsrc(sbox + (long long)zp * d.sz);
+        const rsrc_t rw = plane_rsrc(dst + d.base + d.zorg + (long long)z * d.sz);
+        const int dsb = (int)(d.dstride * (long long)sizeof(S));
         for (int q = (int)threadIdx.x; q < wt.n; q += (int)blockDim.x) {
             const int p = wt.p0 + q;
             const int j = (int)(((unsigned long long)(unsigned)p * d.mag) >> 32);
             const int i = p - j * d.nx;
-            const int o = i + j * d.sy;
-            const Nbr n = neighbours<WRAP>(d, i, j, wt.z);
+            const int xy = d.org_xy + i + j * d.sy;
+            const Nbr n = neighbours<WRAP>(d, i, j, z);
             R f[Q];
-            load_cell<S, GATHER, WRAP>(sp, d.dstride, o, n, f, std::make_integer_sequence<int, Q>{});
+            load_cell_buf<S, GATHER>(rm, r0, rp, dsb, xy, n, f, std::make_integer_sequence<int, Q>{});
             if constexpr (OP != OP_STREAM) {
                 if constexpr (Store<S>::shifted) collide32(f, d.omega_f);
                 else collide64(f, d.omega);
             }
-            store_cell<S, NT>(dp, d.dstride, o, f, std::make_integer_sequence<int, Q>{});
+            store_cell_buf<S, NT>(rw, dsb, xy, f, std::make_integer_sequence<int, Q>{});
         }
     }
 }
@@ -373,16 +491,26 @@ k_halo(const S *__restrict__ src, S *__restrict__ dst, const HaloSeg *__restrict
 }
 
 // ---------------------------------------------------------------------------
-// Plain device copy, 16 bytes per lane, streaming stores: the measured HBM
-// ceiling that the step kernels are compared with.
+// Plain device copy, 16 bytes per lane, four independent loads in flight per
+// lane: the measured HBM ceiling that the step kernels are compared with.
 // ---------------------------------------------------------------------------
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
+template <bool NT>
 __global__ void __launch_bounds__(256)
 k_copy(const u32x4 *__restrict__ src, u32x4 *__restrict__ dst, long long n)
 {
-    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
-        __builtin_nontemporal_store(src[i], dst + i);
+    const long long base = (long long)blockIdx.x * 1024 + threadIdx.x;
+    u32x4 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+        if (base + u * 256 < n) v[u] = src[base + u * 256];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+        if (base + u * 256 < n) {
+            if constexpr (NT) __builtin_nontemporal_store(v[u], dst + base + u * 256);
+            else dst[base + u * 256] = v[u];
+        }
 }
 
 }  // namespace imex

@@ -68,12 +68,18 @@ typedef struct imex_config {
     int cells_per_thread; /* 0 = auto                                     */
     int persistent;       /* 0: one workgroup per work tile; >0: that many
                              workgroups stride over the work-tile table   */
-    int nontemporal;      /* 1: streaming stores bypass the caches        */
+    int nontemporal;      /* 1: streaming (non-temporal) stores, 0: plain
+                             stores, -1 = default (streaming)             */
     int dir_pad;          /* extra elements between direction planes; -1 = auto */
     int use_graph;        /* 1: imex_step replays a captured hipGraph     */
     int sequential;       /* several domains: one launch per domain per
                              step instead of one launch for all           */
     int device;           /* HIP device ordinal                           */
+    int xcd_groups;       /* workgroups are renumbered so each of this many
+                             XCDs takes a contiguous range of work tiles;
+                             0 = 8 (MI355X), 1 = dispatch order           */
+    int contiguous;       /* 1: ask for physically contiguous population
+                             pools (falls back to a plain allocation)     */
 } imex_config;
 
 typedef struct imex_info {
@@ -88,9 +94,12 @@ typedef struct imex_info {
     int block, cells_per_thread, persistent, nontemporal, dir_pad, use_graph;
     int grid;                 /* tile grid rx, ry, rz as setup_Cartesian  */
     int rx, ry, rz;
+    int xcd_groups;
+    int contiguous;           /* 1 if the pools were allocated contiguous */
 } imex_info;
 
-/* Defaults for every field of imex_config (FP64, wrap, autotuned launch). */
+/* Defaults for every field of imex_config: FP64, wrap, 256-thread
+ * workgroups, one cell per thread, streaming stores, XCD-grouped tiles. */
 void imex_config_default(imex_config *cfg);
 
 /* LBM::LBM + the allocation half of LBM::Initialize (src/lbm.hpp:179-183,

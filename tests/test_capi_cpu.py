@@ -44,7 +44,7 @@ def test_strerror_and_defaults():
     assert L.imex_strerror(-2) == b"out of memory"
     c = capi.default_config()
     assert c.precision == capi.F64 and c.mode == capi.WRAP and c.ntiles == 8
-    assert c.halo_depth == 1 and c.dir_pad == -1
+    assert c.halo_depth == 1 and c.dir_pad == -1 and c.nontemporal == -1 and c.xcd_groups == 0
 
 
 @pytest.mark.skipif(gpu_available(), reason="checks the no-GPU error path")
