@@ -24,7 +24,7 @@ EXPORTS = (
     "imex_streaming", "imex_step", "imex_conserved", "imex_get_f", "imex_set_f",
     "imex_synchronize", "imex_timed_steps", "imex_autotune", "imex_query",
     "imex_destroy", "imex_device_count", "imex_strerror", "imex_last_error",
-    "imex_copy_probe",
+    "imex_copy_probe", "imex_get_macro",
 )
 
 
@@ -82,6 +82,7 @@ def load():
         L.imex_destroy.restype = None
         L.imex_device_count.restype = i
         L.imex_copy_probe.argtypes = [i, ctypes.c_longlong, i, dp]
+        L.imex_get_macro.argtypes = [vp, i, dp, dp, dp, dp]
         L.imex_strerror.restype = ctypes.c_char_p
         L.imex_strerror.argtypes = [i]
         L.imex_last_error.restype = ctypes.c_char_p
@@ -183,6 +184,14 @@ class Context:
         out = np.empty((27, d["nx"], d["ny"], d["nz"]), np.float64)
         _check(load().imex_get_f(self._h, int(domain), _dptr(out)))
         return out
+
+    def get_macro(self, domain=0):
+        """ComputeMacroscopic of S_n: (p, u, v, w), each [nx][ny][nz]."""
+        d = self.domains[domain]
+        shape = (d["nx"], d["ny"], d["nz"])
+        out = [np.empty(shape, np.float64) for _ in range(4)]
+        _check(load().imex_get_macro(self._h, int(domain), *(_dptr(a) for a in out)))
+        return tuple(out)
 
     def set_f(self, f, domain=0):
         d = self.domains[domain]

@@ -550,6 +550,24 @@ int launch_export_t(imex_ctx *c, int box, bool gather)
 }
 
 template <class S>
+int launch_macro_t(imex_ctx *c, int box, bool gather)
+{
+    const S *sp = (const S *)pool_ptr(c, c->cur);
+    const BoxDesc &b = c->boxes[box];
+    const long long ncell = (long long)b.nx * b.ny * b.nz;
+    const int grid = (int)std::min<long long>((ncell + 255) / 256, 65536);
+    if (!c->tiled) {
+        if (gather) hipLaunchKernelGGL((k_macro<S, true, true>), dim3(grid), dim3(256), 0, c->stream, sp, c->d_boxes, box, c->d_stage);
+        else hipLaunchKernelGGL((k_macro<S, true, false>), dim3(grid), dim3(256), 0, c->stream, sp, c->d_boxes, box, c->d_stage);
+    } else {
+        if (gather) hipLaunchKernelGGL((k_macro<S, false, true>), dim3(grid), dim3(256), 0, c->stream, sp, c->d_boxes, box, c->d_stage);
+        else hipLaunchKernelGGL((k_macro<S, false, false>), dim3(grid), dim3(256), 0, c->stream, sp, c->d_boxes, box, c->d_stage);
+    }
+    HIPC(hipGetLastError());
+    return IMEX_OK;
+}
+
+template <class S>
 int launch_import_t(imex_ctx *c, int box)
 {
     S *dp = (S *)pool_ptr(c, c->cur);
@@ -872,6 +890,38 @@ int imex_get_f(imex_ctx *c, int domain, double *host)
                         const long long X = b.lo[0] + i, Y = b.lo[1] + j, Z = b.lo[2] + k;
                         host[ii * gcell + (X * NY + Y) * NZ + Z] =
                             tmp[(size_t)ii * ncell + ((size_t)k * b.ny + j) * b.nx + i];
+                    }
+    }
+    return IMEX_OK;
+}
+
+int imex_get_macro(imex_ctx *c, int domain, double *p, double *u, double *v, double *w)
+{
+    TRY(check_ctx(c));
+    TRY(check_domain(c, domain));
+    if (!p || !u || !v || !w) return fail(IMEX_EINVAL, "null output");
+    TRY(flush_pending(c));
+    const bool gather = !c->state_S;
+    if (gather) TRY(ensure_ghosts(c));
+    const Instance &I = c->inst[domain];
+    const long long NY = I.dom.ny, NZ = I.dom.nz;
+    double *outs[4] = {p, u, v, w};
+    std::vector<double> tmp;
+    for (int bi = I.box_begin; bi < I.box_end; ++bi) {
+        const BoxDesc &b = c->boxes[bi];
+        const long long ncell = (long long)b.nx * b.ny * b.nz;
+        TRY(ensure_stage(c, 4 * (size_t)ncell));
+        TRY(DISPATCH(launch_macro_t, c, bi, gather));
+        tmp.resize(4 * (size_t)ncell);
+        HIPC(hipMemcpyAsync(tmp.data(), c->d_stage, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPC(hipStreamSynchronize(c->stream));
+        for (int q = 0; q < 4; ++q)
+            for (int k = 0; k < b.nz; ++k)
+                for (int j = 0; j < b.ny; ++j)
+                    for (int i = 0; i < b.nx; ++i) {
+                        const long long X = b.lo[0] + i, Y = b.lo[1] + j, Z = b.lo[2] + k;
+                        outs[q][(X * NY + Y) * NZ + Z] =
+                            tmp[(size_t)q * ncell + ((size_t)k * b.ny + j) * b.nx + i];
                     }
     }
     return IMEX_OK;

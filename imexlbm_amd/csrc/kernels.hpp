@@ -434,6 +434,40 @@ k_export(const S *__restrict__ src, const BoxDesc *__restrict__ boxes, int box,
     }
 }
 
+// Macroscopic fields of S_n (ComputeMacroscopic, src/lbm.cpp:218-236):
+// p = pl*(1/3), u = ul, v = vl, w = wl from the FP64 moments of the cell, as
+// four planes [field][z][y][x] of doubles.
+template <class S, bool WRAP, bool GATHER>
+__global__ void __launch_bounds__(256)
+k_macro(const S *__restrict__ src, const BoxDesc *__restrict__ boxes, int box,
+        double *__restrict__ out)
+{
+    using R = typename Store<S>::real;
+    const BoxDesc &d = boxes[box];
+    const long long ncell = (long long)d.nx * d.ny * d.nz;
+    for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < ncell;
+         c += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(c % d.nx);
+        const int j = (int)((c / d.nx) % d.ny);
+        const int k = (int)(c / ((long long)d.nx * d.ny));
+        const Nbr n = neighbours<WRAP>(d, i, j, k);
+        R g[Q];
+        load_cell<S, GATHER, WRAP>(src + d.org + (long long)k * d.sz, d.dstride, i + j * d.sy, n, g,
+                                   std::make_integer_sequence<int, Q>{});
+        double f[Q];
+#pragma unroll
+        for (int ii = 0; ii < Q; ++ii) {
+            f[ii] = (double)g[ii];
+            if constexpr (Store<S>::shifted) f[ii] += weight(ii) * d.rho0;
+        }
+        const Moments64 m = moments64(f);
+        out[c] = m.pl * (1.0 / 3.0);
+        out[ncell + c] = m.ul;
+        out[2 * ncell + c] = m.vl;
+        out[3 * ncell + c] = m.wl;
+    }
+}
+
 template <class S>
 __global__ void __launch_bounds__(256)
 k_import(S *__restrict__ dst, const BoxDesc *__restrict__ boxes, int box,

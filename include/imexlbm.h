@@ -133,6 +133,12 @@ int imex_conserved(imex_ctx *ctx, int domain, double out[5]);
  * (ii, i, j, k) order, host[((ii*nx + i)*ny + j)*nz + k].  Blocks. */
 int imex_get_f(imex_ctx *ctx, int domain, double *host);
 
+/* LBM::ComputeMacroscopic (src/lbm.cpp:213-240) for one domain, copied to
+ * the host: p = (Σf)/3 and (u, v, w) = Σf e of S_n, each nx*ny*nz doubles in
+ * the reference's (i, j, k) order.  Blocks. */
+int imex_get_macro(imex_ctx *ctx, int domain, double *p, double *u, double *v,
+                   double *w);
+
 /* Replace one domain's state by S given in the same order (tests use it to
  * drive collision and streaming from arbitrary inputs). */
 int imex_set_f(imex_ctx *ctx, int domain, const double *host);

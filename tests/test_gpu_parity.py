@@ -257,3 +257,53 @@ def test_large_domain_invariants_fp64():
     assert abs(m1[0] - m0[0]) / m0[0] < 1e-12
     assert np.abs(m1[1:4]).max() < 1e-10
     assert 0.9 * m0[4] < m1[4] < 1.01 * m0[4]
+
+
+def test_get_macro_matches_moments_of_oracle_state():
+    from oracle import oracle
+    _, e, _ = oracle.tables()
+    d = _dom(12, ny=10, nz=8)
+    with _ctx(d, precision="fp64") as c:
+        c.initialize(); c.step(5)
+        p, u, v, w = c.get_macro()
+    o = _oracle(d, 5)
+    f = o.get_f()
+    o.close()
+    mom = np.einsum("ia,ixyz->axyz", e.astype(float), f)
+    assert np.abs(p - f.sum(0) / 3.0).max() < 1e-15
+    for got, want in zip((u, v, w), mom):
+        assert np.abs(got - want).max() < 1e-15
+
+
+def test_tiled_256_three_layer_halo_matches_wrap():
+    # the 256^3 deck's decomposition check: 2x2x2 tiles, 3-layer pack/unpack
+    from imexlbm_amd import capi
+    d = _dom(256, re=1600)
+    with _ctx(d, precision="fp64") as w:
+        w.initialize(); w.step(6)
+        cw = w.conserved()
+        fw = w.get_f()
+    with _ctx(d, precision="fp64", mode=capi.TILED, ntiles=8, halo_depth=3) as t:
+        t.initialize(); t.step(6)
+        ct = t.conserved()
+        assert t.info()["ntiles"] == 8
+        assert np.array_equal(fw, t.get_f())
+    del fw
+    assert abs(cw[0] - ct[0]) / cw[0] <= 1e-13 and abs(cw[4] - ct[4]) / cw[4] <= 1e-13
+
+
+def test_precision_study_small():
+    from imexlbm_amd import precision_study
+    recs = precision_study.study(24, 40, 10, 400, 0.05, ["fp32", "fp16", "bf16"])
+    by = {r["prec"]: r for r in recs}
+    assert by["fp32"]["stable"] and by["fp32"]["vel_l2"] < 1e-5
+    assert by["fp16"]["stable"] and by["fp16"]["vel_l2"] < 5e-3
+    assert by["bf16"]["stable"] and by["bf16"]["vel_l2"] < 5e-2
+    assert by["fp64"]["mass_drift"] < 1e-13
+
+
+def test_batch_study_small():
+    from imexlbm_amd import batch_study
+    r = batch_study.run(count=5, steps=20, prec="fp32")
+    assert r["vel_l2_max"] < 1e-5 and r["ke_err_max"] < 1e-5
+    assert r["batched_mlups"] > 0 and r["one_after_another_mlups"] > 0
