@@ -93,6 +93,7 @@ def study(n, steps, every, re, u0, precs, log=sys.stderr):
                         "eps_peak64": eps64, "eps_peak": eps, "t_peak64": t_eps64, "t_peak": t_eps,
                         "eps_peak_err": abs(eps - eps64) / abs(eps64),
                         "mass_drift": float(np.max(np.abs(mass - mass[0]) / mass[0])),
+                        "ke_rel_curve": [float(v) for v in (ke - ke64) / ke64],
                     })
                 del got
             out.append(rec)

@@ -307,3 +307,25 @@ def test_batch_study_small():
     r = batch_study.run(count=5, steps=20, prec="fp32")
     assert r["vel_l2_max"] < 1e-5 and r["ke_err_max"] < 1e-5
     assert r["batched_mlups"] > 0 and r["one_after_another_mlups"] > 0
+
+
+def test_fp32_uniform_flow_keeps_its_momentum():
+    # A uniform flow is a fixed point of stream + collide.  With float-rounded
+    # weights the equilibrium's momentum is u(1+d), d ~ 3e-8, and the flow
+    # would gain ~6e-5 over 1000 steps; with the exact momentum weights only
+    # unbiased rounding remains.
+    from oracle import oracle
+    w, e, _ = oracle.tables()
+    n = 16
+    u = np.array([0.05, -0.03, 0.02])
+    eu = e.astype(float) @ u
+    feq = w * (1.0 + 3.0 * eu + 4.5 * eu ** 2 - 1.5 * (u @ u))
+    f0 = np.broadcast_to(feq[:, None, None, None], (27, n, n, n)).copy()
+    d = {"nx": n, "ny": n, "nz": n, "tau0": 0.002, "rho0": 1.0, "u0": 0.05}
+    with _ctx(d, precision="fp32") as c:
+        c.initialize()
+        c.set_f(f0)
+        p0 = c.conserved()[1:4]
+        c.step(1000)
+        p1 = c.conserved()[1:4]
+    assert np.abs(p1 - p0).max() / np.abs(p0).max() < 2e-6
