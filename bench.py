@@ -41,9 +41,12 @@ def tau0(u0, n, re):
     return u0 * n / re / (1.0 / 3.0) / 2.0 / PI_LIT
 
 
-def run_mode(capi, prec, n, re, u0, steps, warmup, autotune, use_graph):
+def run_mode(capi, prec, n, re, u0, steps, warmup, autotune, use_graph, mode="wrap", tiles=8, halo=1):
     dom = {"nx": n, "ny": n, "nz": n, "tau0": tau0(u0, n, re), "rho0": 1.0, "u0": u0}
-    with capi.Context(dom, precision=prec, use_graph=int(use_graph)) as ctx:
+    kw = {}
+    if mode == "tiled":
+        kw = {"mode": capi.TILED, "ntiles": tiles, "halo_depth": halo}
+    with capi.Context(dom, precision=prec, use_graph=int(use_graph), **kw) as ctx:
         ctx.initialize()
         ctx.step(1)                       # first collision: the state becomes post-collision
         tuned_ms = ctx.autotune(reps=3) if autotune else None
@@ -126,6 +129,10 @@ def main():
                     help="time launch shapes first and keep the fastest")
     ap.add_argument("--graph", action="store_true", help="replay steps from a hipGraph")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--mode", choices=("wrap", "tiled"), default="wrap",
+                    help="periodic wrap in the kernel, or subdomain tiles with device pack/unpack")
+    ap.add_argument("--tiles", type=int, default=8)
+    ap.add_argument("--halo", type=int, default=1, help="ghost layers filled per exchange (tiled)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -145,7 +152,7 @@ def main():
     for m in modes:
         print(f"bench: {m} {args.n}^3 steps={args.steps} warmup={args.warmup}", file=sys.stderr, flush=True)
         results[m] = run_mode(capi, m, args.n, args.re, args.u0, args.steps, args.warmup,
-                              args.autotune, args.graph)
+                              args.autotune, args.graph, args.mode, args.tiles, args.halo)
         print(f"bench: {m} {results[m]['mlups']} MLUPS, {results[m]['achieved_gbs']} GB/s",
               file=sys.stderr, flush=True)
 
@@ -171,7 +178,9 @@ def main():
         "data": "synthetic (analytic Taylor-Green initial field)",
         "config": {
             "workload": f"D3Q27 BGK Taylor-Green vortex {args.n}^3 periodic, Re={args.re}, "
-                        f"u0={args.u0}, rho0=1, fused pull-stream+collide, in-kernel wrap",
+                        f"u0={args.u0}, rho0=1, fused pull-stream+collide, "
+                        + ("in-kernel wrap" if args.mode == "wrap" else
+                           f"{args.tiles} tiles, {args.halo}-layer device pack/unpack"),
             "storage": args.headline,
             "cells": args.n ** 3,
             "parallelism": "single GPU",

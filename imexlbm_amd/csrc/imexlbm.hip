@@ -359,16 +359,24 @@ int launch_step_t(imex_ctx *c, int op, int src, int dst, int t0, int n)
     const int grid = c->persistent > 0 ? std::min(c->persistent, n) : n;
     const dim3 g(grid), b(c->block);
     const bool nt = c->nt != 0;
-#define LAUNCH(W, O, N) hipLaunchKernelGGL((k_step<S, W, O, N>), g, b, 0, c->stream, sp, dp, c->d_boxes, tl, n, c->xcds)
+    // two cells per thread (tiles of 2*block cells or more) use the paired kernel
+    const bool pair = c->cpt >= 2 && op == OP_FUSED;
+#define LAUNCH(W, O, N, P) hipLaunchKernelGGL((k_step<S, W, O, N, P>), g, b, 0, c->stream, sp, dp, c->d_boxes, tl, n, c->xcds)
+#define FUSED(W)                                                              \
+    do {                                                                      \
+        if (pair) { if (nt) LAUNCH(W, OP_FUSED, true, true); else LAUNCH(W, OP_FUSED, false, true); } \
+        else { if (nt) LAUNCH(W, OP_FUSED, true, false); else LAUNCH(W, OP_FUSED, false, false); }   \
+    } while (0)
     if (!c->tiled) {
-        if (op == OP_FUSED) { if (nt) LAUNCH(true, OP_FUSED, true); else LAUNCH(true, OP_FUSED, false); }
-        else if (op == OP_COLLIDE) LAUNCH(true, OP_COLLIDE, false);
-        else LAUNCH(true, OP_STREAM, false);
+        if (op == OP_FUSED) FUSED(true);
+        else if (op == OP_COLLIDE) LAUNCH(true, OP_COLLIDE, false, false);
+        else LAUNCH(true, OP_STREAM, false, false);
     } else {
-        if (op == OP_FUSED) { if (nt) LAUNCH(false, OP_FUSED, true); else LAUNCH(false, OP_FUSED, false); }
-        else if (op == OP_COLLIDE) LAUNCH(false, OP_COLLIDE, false);
-        else LAUNCH(false, OP_STREAM, false);
+        if (op == OP_FUSED) FUSED(false);
+        else if (op == OP_COLLIDE) LAUNCH(false, OP_COLLIDE, false, false);
+        else LAUNCH(false, OP_STREAM, false, false);
     }
+#undef FUSED
 #undef LAUNCH
     HIPC(hipGetLastError());
     return IMEX_OK;
@@ -713,7 +721,7 @@ int imex_create(imex_ctx **out, const imex_config *cfg_in, const imex_domain *do
     c->tiled = cfg.mode == IMEX_TILED;
     c->halo_depth = cfg.halo_depth;
     c->block = cfg.block ? cfg.block : 256;
-    c->cpt = cfg.cells_per_thread ? cfg.cells_per_thread : 1;
+    c->cpt = cfg.cells_per_thread ? cfg.cells_per_thread : 2;
     c->persistent = std::max(cfg.persistent, 0);
     c->nt = cfg.nontemporal >= 0 ? (cfg.nontemporal ? 1 : 0) : 1;
     c->dir_pad = cfg.dir_pad >= 0 ? cfg.dir_pad : 0;

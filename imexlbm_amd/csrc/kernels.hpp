@@ -187,6 +187,30 @@ __device__ inline void load_cell_buf(rsrc_t rm, rsrc_t r0, rsrc_t rp, int dsb, i
                           (xy + (GATHER ? pull_xy<I>(n) : 0)) * (int)sizeof(S), I * dsb)), ...);
 }
 
+// Two cells of a 2-byte format loaded as raw halves packed two to a
+// register (low half: first cell, high half: second cell), widened only
+// when the collision needs them.
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
+template <bool GATHER, int... I>
+__device__ inline void load_pair_raw16(rsrc_t rm, rsrc_t r0, rsrc_t rp, int dsb, int xy1, const Nbr &n1,
+                                       int xy2, const Nbr &n2, u16x2 (&raw)[Q],
+                                       std::integer_sequence<int, I...>)
+{
+    ((raw[I].x = __builtin_amdgcn_raw_buffer_load_b16((!GATHER || EZ[I] == 0) ? r0 : (EZ[I] == 1 ? rm : rp),
+                                                      (xy1 + (GATHER ? pull_xy<I>(n1) : 0)) * 2, I * dsb, 0),
+      raw[I].y = __builtin_amdgcn_raw_buffer_load_b16((!GATHER || EZ[I] == 0) ? r0 : (EZ[I] == 1 ? rm : rp),
+                                                      (xy2 + (GATHER ? pull_xy<I>(n2) : 0)) * 2, I * dsb, 0)),
+     ...);
+}
+
+template <class S> __device__ inline float widen16(unsigned short b);
+template <> __device__ inline float widen16<_Float16>(unsigned short b)
+{
+    return (float)__builtin_bit_cast(_Float16, b);
+}
+template <> __device__ inline float widen16<bf16_t>(unsigned short b) { return bf16_to_float(b); }
+
 template <class S, bool NT, int... I>
 __device__ inline void store_cell_buf(rsrc_t r, int dsb, int xy, const typename Store<S>::real (&f)[Q],
                                       std::integer_sequence<int, I...>)
@@ -210,7 +234,11 @@ __device__ inline int xcd_slot(int w, int g, int x)
     return k * q + min(k, rem) + w / x;
 }
 
-template <class S, bool WRAP, int OP, bool NT>
+// PAIR: each thread takes two cells of its tile, blockDim apart, and issues
+// the loads of both before computing either, which doubles the bytes each
+// wavefront keeps in flight (used for the 2-byte formats, whose 27 loads per
+// cell are otherwise too few to cover HBM latency).
+template <class S, bool WRAP, int OP, bool NT, bool PAIR = false>
 __global__ void __launch_bounds__(512)
 k_step(const S *__restrict__ src, S *__restrict__ dst, const BoxDesc *__restrict__ boxes,
        const WorkTile *__restrict__ tiles, int ntiles, int xcds)
@@ -237,6 +265,44 @@ k_step(const S *__restrict__ src, S *__restrict__ dst, const BoxDesc *__restrict
         const rsrc_t rp = plane_rsrc(sbox + (long long)zp * d.sz);
         const rsrc_t rw = plane_rsrc(dst + d.base + d.zorg + (long long)z * d.sz);
         const int dsb = (int)(d.dstride * (long long)sizeof(S));
+        if constexpr (PAIR) {
+            const int nb = (int)blockDim.x;
+            for (int q = (int)threadIdx.x; q < wt.n; q += 2 * nb) {
+                // the second cell repeats the first when the tile runs out; it
+                // is then stored twice with identical bits by the same lane
+                const int q2 = (q + nb < wt.n) ? q + nb : q;
+                const int p1 = wt.p0 + q, p2 = wt.p0 + q2;
+                const int j1 = (int)(((unsigned long long)(unsigned)p1 * d.mag) >> 32);
+                const int j2 = (int)(((unsigned long long)(unsigned)p2 * d.mag) >> 32);
+                const int i1 = p1 - j1 * d.nx, i2 = p2 - j2 * d.nx;
+                const int xy1 = d.org_xy + i1 + j1 * d.sy, xy2 = d.org_xy + i2 + j2 * d.sy;
+                const Nbr n1 = neighbours<WRAP>(d, i1, j1, z), n2 = neighbours<WRAP>(d, i2, j2, z);
+                R f1[Q], f2[Q];
+                if constexpr (sizeof(S) == 2) {
+                    u16x2 raw[Q];
+                    load_pair_raw16<GATHER>(rm, r0, rp, dsb, xy1, n1, xy2, n2, raw,
+                                            std::make_integer_sequence<int, Q>{});
+                    // keep both cells' loads ahead of any arithmetic or store
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int ii = 0; ii < Q; ++ii) {
+                        f1[ii] = widen16<S>(raw[ii].x);
+                        f2[ii] = widen16<S>(raw[ii].y);
+                    }
+                } else {
+                    load_cell_buf<S, GATHER>(rm, r0, rp, dsb, xy1, n1, f1, std::make_integer_sequence<int, Q>{});
+                    load_cell_buf<S, GATHER>(rm, r0, rp, dsb, xy2, n2, f2, std::make_integer_sequence<int, Q>{});
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+                if constexpr (OP != OP_STREAM) {
+                    if constexpr (Store<S>::shifted) { collide32(f1, d.omega_f); collide32(f2, d.omega_f); }
+                    else { collide64(f1, d.omega); collide64(f2, d.omega); }
+                }
+                store_cell_buf<S, NT>(rw, dsb, xy1, f1, std::make_integer_sequence<int, Q>{});
+                store_cell_buf<S, NT>(rw, dsb, xy2, f2, std::make_integer_sequence<int, Q>{});
+            }
+            continue;
+        }
         for (int q = (int)threadIdx.x; q < wt.n; q += (int)blockDim.x) {
             const int p = wt.p0 + q;
             const int j = (int)(((unsigned long long)(unsigned)p * d.mag) >> 32);

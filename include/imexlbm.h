@@ -65,7 +65,8 @@ typedef struct imex_config {
                              reference's setup_Cartesian; 0 means 8       */
     int halo_depth;       /* IMEX_TILED: ghost layers filled, 1..3; 0 = 1 */
     int block;            /* threads per workgroup, multiple of 64; 0 = auto */
-    int cells_per_thread; /* 0 = auto                                     */
+    int cells_per_thread; /* 0 = 2; 2 or more loads two cells per thread
+                             before computing either                      */
     int persistent;       /* 0: one workgroup per work tile; >0: that many
                              workgroups stride over the work-tile table   */
     int nontemporal;      /* 1: streaming (non-temporal) stores, 0: plain
@@ -99,7 +100,7 @@ typedef struct imex_info {
 } imex_info;
 
 /* Defaults for every field of imex_config: FP64, wrap, 256-thread
- * workgroups, one cell per thread, streaming stores, XCD-grouped tiles. */
+ * workgroups, two cells per thread, streaming stores, XCD-grouped tiles. */
 void imex_config_default(imex_config *cfg);
 
 /* LBM::LBM + the allocation half of LBM::Initialize (src/lbm.hpp:179-183,

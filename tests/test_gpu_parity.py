@@ -329,3 +329,15 @@ def test_fp32_uniform_flow_keeps_its_momentum():
         c.step(1000)
         p1 = c.conserved()[1:4]
     assert np.abs(p1 - p0).max() / np.abs(p0).max() < 2e-6
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "fp16", "bf16"])
+def test_paired_cells_match_single_cells_bitwise(prec):
+    # cells_per_thread >= 2 selects the kernel that loads two cells per thread
+    d = _dom(24, ny=18, nz=7, re=300)
+    out = []
+    for cpt, blk in ((1, 256), (2, 256), (3, 64)):
+        with _ctx(d, precision=prec, cells_per_thread=cpt, block=blk) as c:
+            c.initialize(); c.step(11)
+            out.append(c.get_f())
+    assert np.array_equal(out[0], out[1]) and np.array_equal(out[0], out[2])
