@@ -37,7 +37,7 @@ class Config(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int) for n in (
         "precision", "mode", "ntiles", "halo_depth", "block", "cells_per_thread",
         "persistent", "nontemporal", "dir_pad", "use_graph", "sequential", "device",
-        "xcd_groups", "contiguous")]
+        "xcd_groups", "contiguous", "io16", "xcd_run")]
 
 
 class Info(ctypes.Structure):
@@ -45,7 +45,7 @@ class Info(ctypes.Structure):
                 ("device_bytes", ctypes.c_longlong)] + [(n, ctypes.c_int) for n in (
                     "storage_bytes", "ndomains", "ntiles", "work_tiles", "block",
                     "cells_per_thread", "persistent", "nontemporal", "dir_pad", "use_graph",
-                    "grid", "rx", "ry", "rz", "xcd_groups", "contiguous")]
+                    "grid", "rx", "ry", "rz", "xcd_groups", "contiguous", "io16", "xcd_run")]
 
 
 class ImexError(RuntimeError):

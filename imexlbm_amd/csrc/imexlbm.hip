@@ -140,6 +140,8 @@ struct imex_ctx {
 
     int block = 256, cpt = 1, persistent = 0, nt = 0, dir_pad = 0, halo_depth = 1, xcds = 8;
     int contiguous = 0;
+    int io16 = 0;
+    int xcd_run = 0;
     hipGraph_t graph[2] = {nullptr, nullptr};
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
 
@@ -361,22 +363,29 @@ int launch_step_t(imex_ctx *c, int op, int src, int dst, int t0, int n)
     const bool nt = c->nt != 0;
     // two cells per thread (tiles of 2*block cells or more) use the paired kernel
     const bool pair = c->cpt >= 2 && op == OP_FUSED;
-#define LAUNCH(W, O, N, P) hipLaunchKernelGGL((k_step<S, W, O, N, P>), g, b, 0, c->stream, sp, dp, c->d_boxes, tl, n, c->xcds)
-#define FUSED(W)                                                              \
-    do {                                                                      \
-        if (pair) { if (nt) LAUNCH(W, OP_FUSED, true, true); else LAUNCH(W, OP_FUSED, false, true); } \
-        else { if (nt) LAUNCH(W, OP_FUSED, true, false); else LAUNCH(W, OP_FUSED, false, false); }   \
+    const bool g16 = sizeof(S) == 2 && c->io16 == 1;
+#define LAUNCH(W, O, N, P, G) hipLaunchKernelGGL((k_step<S, W, O, N, P, G>), g, b, 0, c->stream, sp, dp, c->d_boxes, tl, n, c->xcds, c->xcd_run)
+#define FUSED_G(W, G)                                                                                  \
+    do {                                                                                               \
+        if (pair) { if (nt) LAUNCH(W, OP_FUSED, true, true, G); else LAUNCH(W, OP_FUSED, false, true, G); } \
+        else { if (nt) LAUNCH(W, OP_FUSED, true, false, G); else LAUNCH(W, OP_FUSED, false, false, G); }   \
+    } while (0)
+#define FUSED(W)                                                 \
+    do {                                                         \
+        if constexpr (sizeof(S) == 2) { if (g16) FUSED_G(W, true); else FUSED_G(W, false); } \
+        else FUSED_G(W, false);                                  \
     } while (0)
     if (!c->tiled) {
         if (op == OP_FUSED) FUSED(true);
-        else if (op == OP_COLLIDE) LAUNCH(true, OP_COLLIDE, false, false);
-        else LAUNCH(true, OP_STREAM, false, false);
+        else if (op == OP_COLLIDE) LAUNCH(true, OP_COLLIDE, false, false, false);
+        else LAUNCH(true, OP_STREAM, false, false, false);
     } else {
         if (op == OP_FUSED) FUSED(false);
-        else if (op == OP_COLLIDE) LAUNCH(false, OP_COLLIDE, false, false);
-        else LAUNCH(false, OP_STREAM, false, false);
+        else if (op == OP_COLLIDE) LAUNCH(false, OP_COLLIDE, false, false, false);
+        else LAUNCH(false, OP_STREAM, false, false, false);
     }
 #undef FUSED
+#undef FUSED_G
 #undef LAUNCH
     HIPC(hipGetLastError());
     return IMEX_OK;
@@ -646,6 +655,8 @@ void imex_config_default(imex_config *cfg)
     cfg->halo_depth = 1;
     cfg->dir_pad = -1;
     cfg->nontemporal = -1;
+    cfg->xcd_run = -1;
+    cfg->io16 = -1;
 }
 
 const char *imex_strerror(int s)
@@ -725,7 +736,13 @@ int imex_create(imex_ctx **out, const imex_config *cfg_in, const imex_domain *do
     c->persistent = std::max(cfg.persistent, 0);
     c->nt = cfg.nontemporal >= 0 ? (cfg.nontemporal ? 1 : 0) : 1;
     c->dir_pad = cfg.dir_pad >= 0 ? cfg.dir_pad : 0;
-    c->xcds = cfg.xcd_groups > 0 ? cfg.xcd_groups : 8;
+    // Launch defaults per storage width, from same-allocation A/B timings at
+    // 512^3: 8- and 4-byte formats stream fastest when each XCD owns one
+    // contiguous range of tiles (4-byte: runs of 64 tiles), 2-byte formats in
+    // plain dispatch order with global 16-bit loads and stores.
+    c->xcds = cfg.xcd_groups > 0 ? cfg.xcd_groups : (c->ssize == 2 ? 1 : 8);
+    c->xcd_run = cfg.xcd_run >= 0 ? cfg.xcd_run : (c->ssize == 4 ? 64 : 0);
+    c->io16 = cfg.io16 >= 0 ? (cfg.io16 ? 1 : 0) : 1;
     auto bail = [&](int r) { imex_destroy(c); return r; };
 
     if (hipSetDevice(c->device) != hipSuccess) return bail(fail(IMEX_EHIP, "hipSetDevice failed"));
@@ -1006,19 +1023,27 @@ int imex_autotune(imex_ctx *c, int reps, double *best_ms)
     TRY(flush_pending(c));
     TRY(ensure_ghosts(c));
     HIPC(hipStreamSynchronize(c->stream));
-    struct Cand { int block, cpt, pers, nt, xcds; };
+    struct Cand { int block, cpt, pers, nt, xcds, io16, run; };
     std::vector<Cand> cands;
     const int pers_opts[2] = {0, c->ncu * 8};
+    const int io_opts = c->ssize == 2 ? 2 : 1;
     for (int blk : {128, 256, 512})
         for (int cpt : {1, 2, 4})
             for (int pers : pers_opts)
                 for (int nt : {0, 1})
-                    for (int x : {1, 8}) cands.push_back({blk, cpt, pers, nt, x});
-    const Cand orig = {c->block, c->cpt, c->persistent, c->nt, c->xcds};
+                    for (int x : {1, 8})
+                        for (int io = 0; io < io_opts; ++io)
+                            for (int run : {0, 4, 64}) {
+                                if (x == 1 && run) continue;
+                                cands.push_back({blk, cpt, pers, nt, x, io, run});
+                            }
+    const Cand orig = {c->block, c->cpt, c->persistent, c->nt, c->xcds, c->io16, c->xcd_run};
     Cand best = orig;
     double best_t = 1e300;
     for (const Cand &k : cands) {
         c->block = k.block; c->cpt = k.cpt; c->persistent = k.pers; c->nt = k.nt; c->xcds = k.xcds;
+        c->io16 = k.io16;
+        c->xcd_run = k.run;
         build_tiles(c);
         TRY(upload_tiles(c));
         // warm-up, then timed: fused launches cur -> other without swapping,
@@ -1032,12 +1057,14 @@ int imex_autotune(imex_ctx *c, int reps, double *best_ms)
         HIPC(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
         const double t = ms / reps;
         if (std::getenv("IMEXLBM_AUTOTUNE_LOG"))
-            std::fprintf(stderr, "autotune block=%d cpt=%d persistent=%d nt=%d xcds=%d: %.4f ms\n",
-                         k.block, k.cpt, k.pers, k.nt, k.xcds, t);
+            std::fprintf(stderr, "autotune block=%d cpt=%d persistent=%d nt=%d xcds=%d run=%d io16=%d: %.4f ms\n",
+                         k.block, k.cpt, k.pers, k.nt, k.xcds, k.run, k.io16, t);
         if (t < best_t) { best_t = t; best = k; }
     }
     c->block = best.block; c->cpt = best.cpt; c->persistent = best.pers; c->nt = best.nt;
     c->xcds = best.xcds;
+    c->io16 = best.io16;
+    c->xcd_run = best.run;
     build_tiles(c);
     TRY(upload_tiles(c));
     drop_graphs(c);
@@ -1108,6 +1135,8 @@ int imex_query(imex_ctx *c, imex_info *info)
     info->use_graph = c->cfg.use_graph;
     info->xcd_groups = c->xcds;
     info->contiguous = c->contiguous;
+    info->io16 = c->io16;
+    info->xcd_run = c->xcd_run;
     info->grid = c->tiled ? 1 : 0;
     info->rx = c->rgrid[0]; info->ry = c->rgrid[1]; info->rz = c->rgrid[2];
     return IMEX_OK;

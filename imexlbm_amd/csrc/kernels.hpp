@@ -178,32 +178,6 @@ __device__ inline int pull_xy(const Nbr &n)
     return o;
 }
 
-template <class S, bool GATHER, int... I>
-__device__ inline void load_cell_buf(rsrc_t rm, rsrc_t r0, rsrc_t rp, int dsb, int xy, const Nbr &n,
-                                     typename Store<S>::real (&f)[Q], std::integer_sequence<int, I...>)
-{
-    // the source of direction ii lies in plane z - e_z
-    ((f[I] = BufIO<S>::ld((!GATHER || EZ[I] == 0) ? r0 : (EZ[I] == 1 ? rm : rp),
-                          (xy + (GATHER ? pull_xy<I>(n) : 0)) * (int)sizeof(S), I * dsb)), ...);
-}
-
-// Two cells of a 2-byte format loaded as raw halves packed two to a
-// register (low half: first cell, high half: second cell), widened only
-// when the collision needs them.
-typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
-
-template <bool GATHER, int... I>
-__device__ inline void load_pair_raw16(rsrc_t rm, rsrc_t r0, rsrc_t rp, int dsb, int xy1, const Nbr &n1,
-                                       int xy2, const Nbr &n2, u16x2 (&raw)[Q],
-                                       std::integer_sequence<int, I...>)
-{
-    ((raw[I].x = __builtin_amdgcn_raw_buffer_load_b16((!GATHER || EZ[I] == 0) ? r0 : (EZ[I] == 1 ? rm : rp),
-                                                      (xy1 + (GATHER ? pull_xy<I>(n1) : 0)) * 2, I * dsb, 0),
-      raw[I].y = __builtin_amdgcn_raw_buffer_load_b16((!GATHER || EZ[I] == 0) ? r0 : (EZ[I] == 1 ? rm : rp),
-                                                      (xy2 + (GATHER ? pull_xy<I>(n2) : 0)) * 2, I * dsb, 0)),
-     ...);
-}
-
 template <class S> __device__ inline float widen16(unsigned short b);
 template <> __device__ inline float widen16<_Float16>(unsigned short b)
 {
@@ -211,11 +185,90 @@ template <> __device__ inline float widen16<_Float16>(unsigned short b)
 }
 template <> __device__ inline float widen16<bf16_t>(unsigned short b) { return bf16_to_float(b); }
 
-template <class S, bool NT, int... I>
-__device__ inline void store_cell_buf(rsrc_t r, int dsb, int xy, const typename Store<S>::real (&f)[Q],
-                                      std::integer_sequence<int, I...>)
+template <class S> __device__ inline unsigned short narrow16(float v);
+template <> __device__ inline unsigned short narrow16<_Float16>(float v)
 {
-    (BufIO<S>::template st<NT>(f[I], r, xy * (int)sizeof(S), I * dsb), ...);
+    return __builtin_bit_cast(unsigned short, (_Float16)v);        // round to nearest even
+}
+template <> __device__ inline unsigned short narrow16<bf16_t>(float v)
+{
+    return __builtin_bit_cast(unsigned short, (__bf16)v);          // round to nearest even
+}
+
+// The planes one work tile touches: the 27 direction sub-planes of source
+// z-planes z-1, z, z+1 and of destination z-plane z.  Offsets are bytes: the
+// lane's in-plane cell (vo) plus the direction sub-plane (so, wavefront
+// uniform).  4- and 8-byte formats use buffer descriptors, which take the
+// direction offset as a scalar operand.  2-byte formats can instead use global
+// loads and stores with a uniform z-plane base and one 32-bit lane offset
+// (the io16 launch knob): which of the two streams 16-bit data faster is
+// measured, not assumed.
+template <class S, bool GLOBAL> struct Planes;
+
+template <class S> struct Planes<S, false> {
+    using R = typename Store<S>::real;
+    rsrc_t m, c, p, w;
+    __device__ Planes(const S *zm, const S *z0, const S *zp, S *zw)
+        : m(plane_rsrc(zm)), c(plane_rsrc(z0)), p(plane_rsrc(zp)), w(plane_rsrc(zw)) {}
+    // source plane of a direction with z velocity EZV: z - EZV
+    template <int EZV> __device__ R ld(int vo, int so) const
+    {
+        return BufIO<S>::ld(EZV == 0 ? c : (EZV == 1 ? m : p), vo, so);
+    }
+    template <bool NT> __device__ void st(R v, int vo, int so) const { BufIO<S>::template st<NT>(v, w, vo, so); }
+    template <int EZV> __device__ unsigned short raw(int vo, int so) const
+    {
+        return __builtin_amdgcn_raw_buffer_load_b16(EZV == 0 ? c : (EZV == 1 ? m : p), vo, so, 0);
+    }
+};
+
+template <class S> struct Planes<S, true> {
+    using R = float;
+    const char *m, *c, *p;
+    char *w;
+    __device__ Planes(const S *zm, const S *z0, const S *zp, S *zw)
+        : m(reinterpret_cast<const char *>(zm)), c(reinterpret_cast<const char *>(z0)),
+          p(reinterpret_cast<const char *>(zp)), w(reinterpret_cast<char *>(zw)) {}
+    template <int EZV> __device__ unsigned short raw(int vo, int so) const
+    {
+        const char *b = EZV == 0 ? c : (EZV == 1 ? m : p);
+        return *reinterpret_cast<const unsigned short *>(b + (unsigned long long)(unsigned)(so + vo));
+    }
+    template <int EZV> __device__ float ld(int vo, int so) const { return widen16<S>(raw<EZV>(vo, so)); }
+    template <bool NT> __device__ void st(float v, int vo, int so) const
+    {
+        unsigned short *q = reinterpret_cast<unsigned short *>(w + (unsigned long long)(unsigned)(so + vo));
+        const unsigned short b = narrow16<S>(v);
+        if constexpr (NT) __builtin_nontemporal_store(b, q); else *q = b;
+    }
+};
+
+template <class S, bool GATHER, class PL, int... I>
+__device__ inline void load_cell_io(const PL &P, int dsb, int xy, const Nbr &n,
+                                    typename Store<S>::real (&f)[Q], std::integer_sequence<int, I...>)
+{
+    ((f[I] = P.template ld<GATHER ? EZ[I] : 0>((xy + (GATHER ? pull_xy<I>(n) : 0)) * (int)sizeof(S), I * dsb)),
+     ...);
+}
+
+template <class S, bool NT, class PL, int... I>
+__device__ inline void store_cell_io(const PL &P, int dsb, int xy,
+                                     const typename Store<S>::real (&f)[Q], std::integer_sequence<int, I...>)
+{
+    (P.template st<NT>(f[I], xy * (int)sizeof(S), I * dsb), ...);
+}
+
+// Two cells of a 2-byte format loaded as raw halves packed two to a register
+// (low half: first cell, high half: second cell), widened only when needed.
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
+template <class S, bool GATHER, class PL, int... I>
+__device__ inline void load_pair_raw16(const PL &P, int dsb, int xy1, const Nbr &n1, int xy2,
+                                       const Nbr &n2, u16x2 (&raw)[Q], std::integer_sequence<int, I...>)
+{
+    ((raw[I].x = P.template raw<GATHER ? EZ[I] : 0>((xy1 + (GATHER ? pull_xy<I>(n1) : 0)) * 2, I * dsb),
+      raw[I].y = P.template raw<GATHER ? EZ[I] : 0>((xy2 + (GATHER ? pull_xy<I>(n2) : 0)) * 2, I * dsb)),
+     ...);
 }
 
 // ---------------------------------------------------------------------------
@@ -224,28 +277,36 @@ __device__ inline void store_cell_buf(rsrc_t r, int dsb, int xy, const typename 
 // ---------------------------------------------------------------------------
 // Workgroups are dealt to the XCDs round-robin (workgroup w runs on XCD
 // w % X), and each XCD has its own L2.  xcd_slot renumbers workgroups so that
-// the ones sharing an XCD take one contiguous range of work tiles: the cache
-// line an x-shifted direction straddles at a tile boundary is then read from
-// the same L2 by both tiles instead of being fetched from HBM twice.
-__device__ inline int xcd_slot(int w, int g, int x)
+// each XCD takes runs of `run` consecutive work tiles: the cache line an
+// x-shifted direction straddles between two tiles of a run is then read from
+// one L2 instead of being fetched twice, while all XCDs still work on nearby
+// addresses.  run = 0 gives every XCD one contiguous range; the tail that does
+// not fill whole runs keeps dispatch order.
+__device__ inline int xcd_slot(int w, int g, int x, int run)
 {
     if (x <= 1) return w;
-    const int q = g / x, rem = g % x, k = w % x;
-    return k * q + min(k, rem) + w / x;
+    if (run <= 0) {
+        const int q = g / x, rem = g % x, k = w % x;
+        return k * q + min(k, rem) + w / x;
+    }
+    const int full = g / (x * run) * (x * run);
+    if (w >= full) return w;
+    const int idx = w / x;
+    return ((idx / run) * x + w % x) * run + idx % run;
 }
 
 // PAIR: each thread takes two cells of its tile, blockDim apart, and issues
 // the loads of both before computing either, which doubles the bytes each
 // wavefront keeps in flight (used for the 2-byte formats, whose 27 loads per
 // cell are otherwise too few to cover HBM latency).
-template <class S, bool WRAP, int OP, bool NT, bool PAIR = false>
+template <class S, bool WRAP, int OP, bool NT, bool PAIR = false, bool G16 = false>
 __global__ void __launch_bounds__(512)
 k_step(const S *__restrict__ src, S *__restrict__ dst, const BoxDesc *__restrict__ boxes,
-       const WorkTile *__restrict__ tiles, int ntiles, int xcds)
+       const WorkTile *__restrict__ tiles, int ntiles, int xcds, int xcd_run)
 {
     using R = typename Store<S>::real;
     constexpr bool GATHER = (OP != OP_COLLIDE);
-    const int r = xcd_slot((int)blockIdx.x, (int)gridDim.x, xcds);
+    const int r = xcd_slot((int)blockIdx.x, (int)gridDim.x, xcds, xcd_run);
     // one tile per workgroup, or (persistent grid) a contiguous run of tiles
     const int per = ((int)gridDim.x == ntiles) ? 1 : (ntiles + (int)gridDim.x - 1) / (int)gridDim.x;
     const int t_end = min(ntiles, (r + 1) * per);
@@ -260,10 +321,8 @@ k_step(const S *__restrict__ src, S *__restrict__ dst, const BoxDesc *__restrict
             if (zp == d.nz) zp = 0;
         }
         const S *sbox = src + d.base + d.zorg;
-        const rsrc_t r0 = plane_rsrc(sbox + (long long)z * d.sz);
-        const rsrc_t rm = plane_rsrc(sbox + (long long)zm * d.sz);
-        const rsrc_t rp = plane_rsrc(sbox + (long long)zp * d.sz);
-        const rsrc_t rw = plane_rsrc(dst + d.base + d.zorg + (long long)z * d.sz);
+        const Planes<S, G16 && sizeof(S) == 2> P(sbox + (long long)zm * d.sz, sbox + (long long)z * d.sz, sbox + (long long)zp * d.sz,
+                          dst + d.base + d.zorg + (long long)z * d.sz);
         const int dsb = (int)(d.dstride * (long long)sizeof(S));
         if constexpr (PAIR) {
             const int nb = (int)blockDim.x;
@@ -280,8 +339,7 @@ k_step(const S *__restrict__ src, S *__restrict__ dst, const BoxDesc *__restrict
                 R f1[Q], f2[Q];
                 if constexpr (sizeof(S) == 2) {
                     u16x2 raw[Q];
-                    load_pair_raw16<GATHER>(rm, r0, rp, dsb, xy1, n1, xy2, n2, raw,
-                                            std::make_integer_sequence<int, Q>{});
+                    load_pair_raw16<S, GATHER>(P, dsb, xy1, n1, xy2, n2, raw, std::make_integer_sequence<int, Q>{});
                     // keep both cells' loads ahead of any arithmetic or store
                     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -290,16 +348,16 @@ k_step(const S *__restrict__ src, S *__restrict__ dst, const BoxDesc *__restrict
                         f2[ii] = widen16<S>(raw[ii].y);
                     }
                 } else {
-                    load_cell_buf<S, GATHER>(rm, r0, rp, dsb, xy1, n1, f1, std::make_integer_sequence<int, Q>{});
-                    load_cell_buf<S, GATHER>(rm, r0, rp, dsb, xy2, n2, f2, std::make_integer_sequence<int, Q>{});
+                    load_cell_io<S, GATHER>(P, dsb, xy1, n1, f1, std::make_integer_sequence<int, Q>{});
+                    load_cell_io<S, GATHER>(P, dsb, xy2, n2, f2, std::make_integer_sequence<int, Q>{});
                     __builtin_amdgcn_sched_barrier(0);
                 }
                 if constexpr (OP != OP_STREAM) {
                     if constexpr (Store<S>::shifted) { collide32(f1, d.omega_f); collide32(f2, d.omega_f); }
                     else { collide64(f1, d.omega); collide64(f2, d.omega); }
                 }
-                store_cell_buf<S, NT>(rw, dsb, xy1, f1, std::make_integer_sequence<int, Q>{});
-                store_cell_buf<S, NT>(rw, dsb, xy2, f2, std::make_integer_sequence<int, Q>{});
+                store_cell_io<S, NT>(P, dsb, xy1, f1, std::make_integer_sequence<int, Q>{});
+                store_cell_io<S, NT>(P, dsb, xy2, f2, std::make_integer_sequence<int, Q>{});
             }
             continue;
         }
@@ -310,12 +368,12 @@ k_step(const S *__restrict__ src, S *__restrict__ dst, const BoxDesc *__restrict
             const int xy = d.org_xy + i + j * d.sy;
             const Nbr n = neighbours<WRAP>(d, i, j, z);
             R f[Q];
-            load_cell_buf<S, GATHER>(rm, r0, rp, dsb, xy, n, f, std::make_integer_sequence<int, Q>{});
+            load_cell_io<S, GATHER>(P, dsb, xy, n, f, std::make_integer_sequence<int, Q>{});
             if constexpr (OP != OP_STREAM) {
                 if constexpr (Store<S>::shifted) collide32(f, d.omega_f);
                 else collide64(f, d.omega);
             }
-            store_cell_buf<S, NT>(rw, dsb, xy, f, std::make_integer_sequence<int, Q>{});
+            store_cell_io<S, NT>(P, dsb, xy, f, std::make_integer_sequence<int, Q>{});
         }
     }
 }

@@ -77,10 +77,18 @@ typedef struct imex_config {
                              step instead of one launch for all           */
     int device;           /* HIP device ordinal                           */
     int xcd_groups;       /* workgroups are renumbered so each of this many
-                             XCDs takes a contiguous range of work tiles;
-                             0 = 8 (MI355X), 1 = dispatch order           */
+                             XCDs takes its own work tiles (see xcd_run);
+                             1 = dispatch order; 0 = auto: 8 (MI355X) for
+                             8- and 4-byte formats, 1 for 2-byte ones     */
     int contiguous;       /* 1: ask for physically contiguous population
                              pools (falls back to a plain allocation)     */
+    int io16;             /* 2-byte formats: 0 = buffer-descriptor loads
+                             and stores, 1 = global loads and stores;
+                             -1 = auto (1)                                */
+    int xcd_run;          /* with xcd_groups > 1: each XCD takes runs of
+                             this many consecutive work tiles; 0 = one
+                             contiguous range per XCD; -1 = auto (64 for
+                             4-byte formats, else 0)                      */
 } imex_config;
 
 typedef struct imex_info {
@@ -97,10 +105,13 @@ typedef struct imex_info {
     int rx, ry, rz;
     int xcd_groups;
     int contiguous;           /* 1 if the pools were allocated contiguous */
+    int io16;
+    int xcd_run;
 } imex_info;
 
 /* Defaults for every field of imex_config: FP64, wrap, 256-thread
- * workgroups, two cells per thread, streaming stores, XCD-grouped tiles. */
+ * workgroups, two cells per thread, streaming stores, and the per-format
+ * automatic tile order and 16-bit access path. */
 void imex_config_default(imex_config *cfg);
 
 /* LBM::LBM + the allocation half of LBM::Initialize (src/lbm.hpp:179-183,

@@ -341,3 +341,16 @@ def test_paired_cells_match_single_cells_bitwise(prec):
             c.initialize(); c.step(11)
             out.append(c.get_f())
     assert np.array_equal(out[0], out[1]) and np.array_equal(out[0], out[2])
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp16", "bf16"])
+def test_tile_order_and_access_path_do_not_change_results(prec):
+    d = _dom(20, ny=16, nz=6, re=300)
+    out = []
+    for cfg in ({}, {"io16": 0}, {"io16": 1}, {"xcd_groups": 8, "xcd_run": 3},
+                {"xcd_groups": 8, "xcd_run": 0}, {"xcd_groups": 1}):
+        with _ctx(d, precision=prec, **cfg) as c:
+            c.initialize(); c.step(9)
+            out.append(c.get_f())
+    for o in out[1:]:
+        assert np.array_equal(out[0], o)
