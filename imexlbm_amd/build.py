@@ -22,7 +22,7 @@ ARCH = os.environ.get("IMEXLBM_ARCH", "gfx950")
 HIP_SOURCES = [os.path.join(HERE, "csrc", "imexlbm.hip")]
 HIP_DEPS = HIP_SOURCES + [os.path.join(HERE, "csrc", f) for f in ("d3q27.hpp", "kernels.hpp")] + [
     os.path.join(ROOT, "include", "imexlbm.h")]
-HOST_SOURCES = [os.path.join(HERE, "host", f) for f in ("System.cpp", "lbm.cpp", "main.cpp")]
+HOST_SOURCES = [os.path.join(HERE, "host", f) for f in ("System.cpp", "lbm.cpp", "output.cpp", "main.cpp")]
 
 # -ffp-contract=off: the FP64 path reproduces the reference's separately
 # rounded multiplies and adds; the FP32 path writes its fused multiply-adds

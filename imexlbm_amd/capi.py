@@ -24,7 +24,8 @@ EXPORTS = (
     "imex_streaming", "imex_step", "imex_conserved", "imex_get_f", "imex_set_f",
     "imex_synchronize", "imex_timed_steps", "imex_autotune", "imex_query",
     "imex_destroy", "imex_device_count", "imex_strerror", "imex_last_error",
-    "imex_copy_probe", "imex_get_macro",
+    "imex_copy_probe", "imex_get_macro", "imex_profile_phase_name", "imex_profile_reset",
+    "imex_profile_read",
 )
 
 
@@ -37,7 +38,7 @@ class Config(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int) for n in (
         "precision", "mode", "ntiles", "halo_depth", "block", "cells_per_thread",
         "persistent", "nontemporal", "dir_pad", "use_graph", "sequential", "device",
-        "xcd_groups", "contiguous", "io16", "xcd_run")]
+        "xcd_groups", "contiguous", "io16", "profile", "xcd_run")]
 
 
 class Info(ctypes.Structure):
@@ -83,6 +84,10 @@ def load():
         L.imex_device_count.restype = i
         L.imex_copy_probe.argtypes = [i, ctypes.c_longlong, i, dp]
         L.imex_get_macro.argtypes = [vp, i, dp, dp, dp, dp]
+        L.imex_profile_phase_name.argtypes = [i]
+        L.imex_profile_phase_name.restype = ctypes.c_char_p
+        L.imex_profile_reset.argtypes = [vp]
+        L.imex_profile_read.argtypes = [vp, i, dp, ctypes.POINTER(ctypes.c_longlong)]
         L.imex_strerror.restype = ctypes.c_char_p
         L.imex_strerror.argtypes = [i]
         L.imex_last_error.restype = ctypes.c_char_p
@@ -210,6 +215,24 @@ class Context:
         b = np.zeros(1, np.float64)
         _check(load().imex_autotune(self._h, int(reps), _dptr(b)))
         return float(b[0])
+
+    def profile(self):
+        """{phase name: (calls, total ms, fastest ms, slowest ms, bytes)}"""
+        L = load()
+        out = {}
+        k = 0
+        while True:
+            name = L.imex_profile_phase_name(k)
+            if name is None:
+                return out
+            v = np.zeros(4, np.float64)
+            n = ctypes.c_longlong()
+            _check(L.imex_profile_read(self._h, k, _dptr(v), ctypes.byref(n)))
+            out[name.decode()] = (n.value, *map(float, v))
+            k += 1
+
+    def profile_reset(self):
+        _check(load().imex_profile_reset(self._h))
 
     def info(self):
         i = Info()

@@ -96,6 +96,17 @@ void cartesian(int n, const int gl[3], int r[3])
 int block_len(int gl, int r, int p) { return (p - gl % r >= 0) ? gl / r : gl / r + 1; }
 int block_lo(int gl, int r, int p) { return p * (gl / r) + std::min(p, gl % r); }
 
+// Phases of the per-phase profiler (the counterpart of the reference's
+// LBM_PROFILE build, src/profile.hpp:42-113).
+enum Phase { PH_COLLIDE = 0, PH_FUSED, PH_STREAM, PH_HALO, PH_CONSERVED, PH_MACRO, PH_N };
+const char *const kPhaseNames[PH_N] = {"collide in place (first step)", "fused pull-stream + collide",
+                                       "bare stream", "ghost fill (pack + unpack)", "conserved sums",
+                                       "macroscopic fields"};
+struct PhaseStat {
+    double ms = 0, mn = 0, mx = 0, bytes = 0;
+    long long calls = 0;
+};
+
 }  // namespace
 
 struct imex_ctx {
@@ -147,7 +158,45 @@ struct imex_ctx {
 
     long long cells = 0;
     long long device_bytes = 0;
+    long long halo_bytes = 0;    // bytes one pack + unpack moves
+
+    bool profile = false;        // time every phase with events (adds a sync)
+    bool capturing = false;      // inside a hipGraph capture: no timing
+    PhaseStat ph[PH_N];
+    hipEvent_t pev[2] = {nullptr, nullptr};
 };
+
+namespace {
+
+// Times one phase with a pair of events and a synchronisation when the
+// profiler is on; a no-op otherwise.
+struct PhaseScope {
+    imex_ctx *c;
+    int p;
+    double bytes;
+    bool on;
+    PhaseScope(imex_ctx *c_, int p_, double b) : c(c_), p(p_), bytes(b), on(c_->profile && !c_->capturing)
+    {
+        if (on) (void)hipEventRecord(c->pev[0], c->stream);
+    }
+    ~PhaseScope()
+    {
+        if (!on) return;
+        (void)hipEventRecord(c->pev[1], c->stream);
+        float ms = 0.f;
+        if (hipEventSynchronize(c->pev[1]) != hipSuccess ||
+            hipEventElapsedTime(&ms, c->pev[0], c->pev[1]) != hipSuccess)
+            return;
+        PhaseStat &st = c->ph[p];
+        st.mn = st.calls ? std::min(st.mn, (double)ms) : ms;
+        st.mx = st.calls ? std::max(st.mx, (double)ms) : ms;
+        st.ms += ms;
+        st.bytes += bytes;
+        ++st.calls;
+    }
+};
+
+}  // namespace
 
 namespace {
 
@@ -347,6 +396,9 @@ void build_halo(imex_ctx *c)
         for (int c0 = 0; c0 < unpack[s].ncell; c0 += 256) c->unpack_blocks.push_back({npack + s, c0});
     c->segs.insert(c->segs.end(), unpack.begin(), unpack.end());
     c->xbuf_elems = xoff;
+    c->halo_bytes = 0;
+    for (const HaloSeg &sg : c->segs)   // each segment reads and writes its cells
+        c->halo_bytes += 2LL * __builtin_popcount(sg.mask) * sg.ncell * c->ssize;
 }
 
 // --------------------------------------------------------------------------
@@ -393,6 +445,8 @@ int launch_step_t(imex_ctx *c, int op, int src, int dst, int t0, int n)
 
 int launch_step(imex_ctx *c, int op, int src, int dst)
 {
+    PhaseScope ps(c, op == OP_FUSED ? PH_FUSED : (op == OP_COLLIDE ? PH_COLLIDE : PH_STREAM),
+                  2.0 * Q * c->ssize * (double)c->cells);
     auto one = [&](int t0, int n) -> int {
         if (n <= 0) return IMEX_OK;
         switch (c->prec) {
@@ -424,6 +478,7 @@ int launch_halo_t(imex_ctx *c, int which)
 int ensure_ghosts(imex_ctx *c)
 {
     if (!c->tiled || c->ghosts_valid) return IMEX_OK;
+    PhaseScope ps(c, PH_HALO, (double)c->halo_bytes);
     switch (c->prec) {
     case IMEX_F64: TRY(launch_halo_t<double>(c, c->cur)); break;
     case IMEX_F32: TRY(launch_halo_t<float>(c, c->cur)); break;
@@ -498,8 +553,10 @@ int build_graph(imex_ctx *c, int parity)
     c->cur = parity;
     c->ghosts_valid = false;
     HIPC(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    c->capturing = true;
     int r = do_fused(c);
     if (r == IMEX_OK) r = do_fused(c);
+    c->capturing = false;
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(c->stream, &g);
     c->cur = save_cur;
@@ -517,7 +574,7 @@ int build_graph(imex_ctx *c, int parity)
 int run_steps(imex_ctx *c, int nsteps)
 {
     while (nsteps > 0) {
-        if (c->cfg.use_graph && nsteps >= 2 && !c->state_S && !c->pending) {
+        if (c->cfg.use_graph && !c->profile && nsteps >= 2 && !c->state_S && !c->pending) {
             if (!c->gexec[c->cur]) TRY(build_graph(c, c->cur));
             HIPC(hipGraphLaunch(c->gexec[c->cur], c->stream));
             c->ghosts_valid = false;
@@ -695,6 +752,8 @@ void imex_destroy(imex_ctx *c)
     if (c->h_out) (void)hipHostFree(c->h_out);
     for (hipEvent_t e : c->ev)
         if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : c->pev)
+        if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -751,6 +810,9 @@ int imex_create(imex_ctx **out, const imex_config *cfg_in, const imex_domain *do
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return bail(fail(IMEX_EHIP, "stream creation failed"));
     for (hipEvent_t &e : c->ev)
         if (hipEventCreate(&e) != hipSuccess) return bail(fail(IMEX_EHIP, "event creation failed"));
+    for (hipEvent_t &e : c->pev)
+        if (hipEventCreate(&e) != hipSuccess) return bail(fail(IMEX_EHIP, "event creation failed"));
+    c->profile = cfg.profile != 0;
 
     // instances and their init tables
     int tab = 0;
@@ -881,6 +943,7 @@ int imex_conserved(imex_ctx *c, int domain, double out[5])
     const bool gather = !c->state_S;
     if (gather) TRY(ensure_ghosts(c));
     const Instance &I = c->inst[domain];
+    PhaseScope ps(c, PH_CONSERVED, (double)Q * c->ssize * I.dom.nx * (double)I.dom.ny * I.dom.nz);
     TRY(DISPATCH(launch_conserved_t, c, I.tile_begin, I.tile_end - I.tile_begin, gather));
     HIPC(hipMemcpyAsync(c->h_out, c->d_out, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPC(hipStreamSynchronize(c->stream));
@@ -936,7 +999,10 @@ int imex_get_macro(imex_ctx *c, int domain, double *p, double *u, double *v, dou
         const BoxDesc &b = c->boxes[bi];
         const long long ncell = (long long)b.nx * b.ny * b.nz;
         TRY(ensure_stage(c, 4 * (size_t)ncell));
-        TRY(DISPATCH(launch_macro_t, c, bi, gather));
+        {
+            PhaseScope ps(c, PH_MACRO, (Q * c->ssize + 32.0) * (double)ncell);
+            TRY(DISPATCH(launch_macro_t, c, bi, gather));
+        }
         tmp.resize(4 * (size_t)ncell);
         HIPC(hipMemcpyAsync(tmp.data(), c->d_stage, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIPC(hipStreamSynchronize(c->stream));
@@ -1114,6 +1180,31 @@ int imex_copy_probe(int device, long long bytes, int reps, double *gbs)
     (void)hipFree(a);
     (void)hipFree(b);
     return r;
+}
+
+const char *imex_profile_phase_name(int phase)
+{
+    return (phase >= 0 && phase < PH_N) ? kPhaseNames[phase] : nullptr;
+}
+
+int imex_profile_reset(imex_ctx *c)
+{
+    if (!c) return fail(IMEX_EINVAL, "null context");
+    for (PhaseStat &p : c->ph) p = PhaseStat();
+    return IMEX_OK;
+}
+
+int imex_profile_read(imex_ctx *c, int phase, double out[4], long long *calls)
+{
+    if (!c || !out || !calls) return fail(IMEX_EINVAL, "null argument");
+    if (phase < 0 || phase >= PH_N) return fail(IMEX_EINVAL, "phase out of range");
+    const PhaseStat &p = c->ph[phase];
+    out[0] = p.ms;
+    out[1] = p.mn;
+    out[2] = p.mx;
+    out[3] = p.bytes;
+    *calls = p.calls;
+    return IMEX_OK;
 }
 
 int imex_query(imex_ctx *c, imex_info *info)

@@ -51,7 +51,7 @@ LBM::LBM(lbm_comm comm_, int sx, int sy, int sz, double &tau, double &rho0_, dou
     }
     cfg.ntiles = env_int("IMEXLBM_TILES", 8);
     cfg.halo_depth = env_int("IMEXLBM_HALO", 1);
-    cfg.use_graph = env_int("IMEXLBM_GRAPH", 0);
+    cfg.profile = env_int("IMEXLBM_PROFILE", 0);
 
     // one rank owns the whole domain (src/mpi_view_transfer.cpp:163-195 at rx=ry=rz=1)
     l_l[0] = glx; l_l[1] = gly; l_l[2] = glz;
@@ -93,3 +93,41 @@ void LBM::Conserved(double &mass, double &mom_x, double &mom_y, double &mom_z, d
 void LBM::fence() { check(imex_synchronize(ctx), "imex_synchronize"); }
 
 void LBM::download_f(double *host) { check(imex_get_f(ctx, 0, host), "imex_get_f"); }
+
+void LBM::ProfileReset()
+{
+    if (cfg.profile) check(imex_profile_reset(ctx), "imex_profile_reset");
+}
+
+// Table in the spirit of src/profile.cpp:99-204: per phase the mean time per
+// timed step, the fastest and slowest call, the share of the step, and the
+// algorithmic bandwidth against a copy ceiling measured on this device
+// (where the reference assumes 1600 GB/s).
+void LBM::ProfileReport(int nsteps)
+{
+    if (!cfg.profile || nsteps <= 0) return;
+    double peak = 0.0;
+    if (imex_copy_probe(cfg.device, 4LL << 30, 5, &peak) != IMEX_OK) peak = 0.0;
+    std::printf("\n=== per-phase profile: 1 GPU, %d timed steps ===\n", nsteps);
+    std::printf("%-34s %6s %10s %9s %9s %7s %9s %7s\n", "phase", "calls", "ms/step", "min ms", "max ms",
+                "step%", "GB/s", "%copy");
+    double step_ms = 0.0;
+    for (int p = 0; imex_profile_phase_name(p); ++p) {
+        double v[4];
+        long long n;
+        check(imex_profile_read(ctx, p, v, &n), "imex_profile_read");
+        step_ms += v[0];
+    }
+    for (int p = 0; imex_profile_phase_name(p); ++p) {
+        double v[4];
+        long long n;
+        check(imex_profile_read(ctx, p, v, &n), "imex_profile_read");
+        if (n == 0) continue;
+        const double gbs = v[0] > 0 ? v[3] / (v[0] * 1e-3) / 1e9 : 0.0;
+        std::printf("%-34s %6lld %10.4f %9.4f %9.4f %7.1f %9.1f %7.1f\n", imex_profile_phase_name(p), n,
+                    v[0] / nsteps, v[1], v[2], step_ms > 0 ? 100.0 * v[0] / step_ms : 0.0, gbs,
+                    peak > 0 ? 100.0 * gbs / peak : 0.0);
+    }
+    std::printf("copy ceiling measured on this device: %.1f GB/s\n", peak);
+    std::printf("profile run (one synchronisation per phase): take throughput from the MLUPS lines\n\n");
+}

@@ -19,9 +19,12 @@
 //   IMEXLBM_TILES     = subdomain count in tiled mode (default 8)
 //   IMEXLBM_HALO      = ghost layers filled per exchange, 1..3 (default 1)
 //   IMEXLBM_AUTOTUNE  = 1 to time launch shapes after Initialize
-//   IMEXLBM_GRAPH     = 1 to replay steps from a captured hipGraph
+//   IMEXLBM_OUTPUT    = 1 to write output<n>.plt at every report
+//   IMEXLBM_PROFILE   = 1 for the per-phase profile table at the end
 #ifndef IMEXLBM_LBM_HPP
 #define IMEXLBM_LBM_HPP
+
+#include <vector>
 
 #include "imexlbm.h"
 
@@ -63,6 +66,18 @@ struct LBM
     void Collision();
     void Streaming();
     void Conserved(double &mass, double &mom_x, double &mom_y, double &mom_z, double &ke);
+
+    // Field output (src/lbm.cpp:213-240, 308-636): p, u, v, w of the current
+    // state on the host, a Tecplot binary "output<n>.plt" and the ASCII
+    // "output<n><me>.dat" of the rest population.
+    void ComputeMacroscopic();
+    void MPIoutput(int n);
+    void Output(int n);
+    std::vector<double> p_h, u_h, v_h, w_h;   // (i, j, k), k fastest
+
+    // Per-phase profile (the reference's LBM_PROF_RESET / LBM_PROF_REPORT).
+    void ProfileReset();
+    void ProfileReport(int nsteps);
 
     // Device synchronisation, the counterpart of Kokkos::fence().
     void fence();

@@ -6,6 +6,7 @@
 // Timing follows the reference's scaling build (no per-step fences).
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 
 #include "System.hpp"
 #include "lbm.hpp"
@@ -33,6 +34,8 @@ int main()
         {
             l1.Collision();
             l1.Streaming();
+            // step 1 carries the one-time first collision (src/main.cpp:66-68)
+            if (it == 1) l1.ProfileReset();
 
             if (it % s1.inter == 0)
             {
@@ -54,9 +57,13 @@ int main()
                                 it, mass, px, py, pz, ke);
                     std::fflush(stdout);
                 }
+                // the reference leaves this call commented out (src/main.cpp:99)
+                if (std::getenv("IMEXLBM_OUTPUT") && std::getenv("IMEXLBM_OUTPUT")[0] == '1')
+                    l1.MPIoutput(it / s1.inter);
                 t_last = wtime();
             }
         }
+        l1.ProfileReport(s1.Time - 1);
     }
     return 0;
 }

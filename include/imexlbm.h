@@ -85,6 +85,8 @@ typedef struct imex_config {
     int io16;             /* 2-byte formats: 0 = buffer-descriptor loads
                              and stores, 1 = global loads and stores;
                              -1 = auto (1)                                */
+    int profile;          /* 1: time every phase with HIP events (one
+                             synchronisation per phase, graphs off)       */
     int xcd_run;          /* with xcd_groups > 1: each XCD takes runs of
                              this many consecutive work tiles; 0 = one
                              contiguous range per XCD; -1 = auto (64 for
@@ -170,6 +172,16 @@ int imex_timed_steps(imex_ctx *ctx, int nsteps, double *wall_ms,
 int imex_autotune(imex_ctx *ctx, int reps, double *best_ms);
 
 int imex_query(imex_ctx *ctx, imex_info *info);
+
+/* Per-phase profiler, the counterpart of the reference's LBM_PROFILE build
+ * (src/profile.hpp:42-113, src/profile.cpp:99-204).  With cfg.profile set,
+ * each phase (first collide, fused step, bare stream, ghost fill, conserved
+ * sums, macroscopic fields) is timed with HIP events.  imex_profile_read
+ * gives out = {total ms, fastest ms, slowest ms, algorithmic bytes} and the
+ * call count; imex_profile_phase_name returns NULL past the last phase. */
+const char *imex_profile_phase_name(int phase);
+int imex_profile_reset(imex_ctx *ctx);
+int imex_profile_read(imex_ctx *ctx, int phase, double out[4], long long *calls);
 
 /* Measured HBM ceiling: a plain device-to-device copy of `bytes` (16 bytes
  * per lane, streaming stores), timed over `reps` launches; *gbs receives read

@@ -74,3 +74,32 @@ def test_driver_aborts_without_deck(tmp_path):
     r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True)
     assert r.returncode != 0
     assert "cannot open input.in" in r.stderr
+
+
+def _struct_fields(name):
+    txt = open(os.path.join(ROOT, "include", "imexlbm.h")).read()
+    body = re.search(r"typedef struct %s \{(.*?)\} %s;" % (name, name), txt, re.S).group(1)
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    fields = []
+    for decl in body.split(";"):
+        decl = decl.strip()
+        if not decl:
+            continue
+        typ, names = re.match(r"(long long|int|double)\s+(.*)", decl).groups()
+        for n in names.split(","):
+            fields.append((n.strip(), typ))
+    return fields
+
+
+@pytest.mark.parametrize("cname,pyname", [("imex_config", "Config"), ("imex_info", "Info"),
+                                          ("imex_domain", "Domain")])
+def test_ctypes_structs_match_header(cname, pyname):
+    import ctypes
+    from imexlbm_amd import capi
+    want = _struct_fields(cname)
+    if cname == "imex_domain":
+        want = [(n, t) for n, t in want]
+    got = getattr(capi, pyname)._fields_
+    cmap = {"int": ctypes.c_int, "long long": ctypes.c_longlong, "double": ctypes.c_double}
+    assert [n for n, _ in got] == [n for n, _ in want]
+    assert all(t == cmap[w] for (_, t), (_, w) in zip(got, want))

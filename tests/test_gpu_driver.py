@@ -72,3 +72,69 @@ def test_driver_autotune_matches_plain(tmp_path):
     # when the tuned tile size changes their order
     for a, b in zip(ref, got):
         assert abs(b[1] - a[1]) / a[1] <= 1e-14 and abs(b[5] - a[5]) / a[5] <= 1e-14
+
+
+def _read_plt(path, n):
+    import numpy as np
+    raw = open(path, "rb").read()
+    assert raw[:8] == b"#!TDV112"
+    hdr = np.frombuffer(raw[8:332], dtype=np.uint8)
+    i32 = lambda off: int(np.frombuffer(raw[off:off + 4], "<i4")[0])
+    f32 = lambda off: float(np.frombuffer(raw[off:off + 4], "<f4")[0])
+    assert (i32(8), i32(12), i32(16), i32(20)) == (1, 0, 0, 7)
+    names = "".join(chr(i32(24 + 8 * k)) for k in range(7))
+    assert names == "xyzuvwp" and all(i32(28 + 8 * k) == 0 for k in range(7))
+    assert f32(80) == 299.0
+    assert "".join(chr(i32(84 + 4 * k)) for k in range(8)) == "ZONE 001"
+    assert float(np.frombuffer(raw[124:132], "<f8")[0]) == float(n)
+    dims = (i32(156), i32(160), i32(164))
+    assert f32(172) == 357.0 and f32(176) == 299.0
+    assert [i32(180 + 4 * k) for k in range(7)] == [2] * 7
+    mm = np.frombuffer(raw[220:332], "<f8").reshape(7, 2)
+    data = np.frombuffer(raw[332:], "<f8")
+    nx, ny, nz = dims
+    assert data.size == 7 * nx * ny * nz
+    arrs = data.reshape(7, nz, ny, nx)          # Fortran order: x fastest
+    assert len(hdr) == 324
+    return dims, mm, arrs
+
+
+def test_driver_tecplot_output_matches_macroscopic_fields(tmp_path):
+    import numpy as np
+    from oracle import oracle
+    deck = "1.0  0.1  400\n0.05  6  3\n12  10  8\n"
+    _run(tmp_path, deck=deck, IMEXLBM_OUTPUT=1)
+    tau = oracle.tau(0.05, 12, 400)
+    o = oracle.Oracle(12, 10, 8, tau, 1.0, 0.05)
+    o.initialize()
+    _, e, _ = oracle.tables()
+    for n in (1, 2):
+        o.step(3)
+        f = o.get_f()
+        p = f.sum(0) / 3.0
+        uvw = np.einsum("ia,ixyz->axyz", e.astype(float), f)
+        dims, mm, arrs = _read_plt(tmp_path / f"output{n}.plt", n)
+        assert dims == (12, 10, 8)
+        x, y, z = (a.transpose(2, 1, 0) for a in arrs[:3])      # -> [i][j][k]
+        assert np.allclose(x[:, 0, 0], np.arange(12) / 11.0, rtol=0, atol=0)
+        assert np.allclose(y[0, :, 0], np.arange(10) / 9.0, rtol=0, atol=0)
+        assert np.allclose(z[0, 0, :], np.arange(8) / 7.0, rtol=0, atol=0)
+        for q, want in enumerate((uvw[0], uvw[1], uvw[2], p)):
+            got = arrs[3 + q].transpose(2, 1, 0)
+            assert np.abs(got - want).max() < 1e-15
+            assert mm[3 + q, 0] == got.min() and mm[3 + q, 1] == got.max()
+    o.close()
+
+
+def test_driver_phase_profile(tmp_path):
+    ref, _ = _run(tmp_path)
+    got, out = _run(tmp_path, IMEXLBM_PROFILE=1, IMEXLBM_MODE="tiled", IMEXLBM_TILES=8)
+    assert "=== per-phase profile: 1 GPU, 19 timed steps ===" in out
+    rows = {l[:34].strip(): l[34:].split() for l in out.splitlines()
+            if l.startswith(("fused pull-stream", "ghost fill", "conserved sums"))}
+    assert int(rows["fused pull-stream + collide"][0]) == 19
+    assert int(rows["ghost fill (pack + unpack)"][0]) == 19
+    assert int(rows["conserved sums"][0]) == 19
+    assert float(rows["fused pull-stream + collide"][5]) > 0      # GB/s
+    for a, b in zip(ref, got):
+        assert abs(b[1] - a[1]) / a[1] <= 1e-12 and abs(b[5] - a[5]) / a[5] <= 1e-12
