@@ -7,7 +7,9 @@ fused pull-stream + BGK-collide kernel with in-kernel periodic wrap.  One
 "step" is one lattice update of every cell.  Each storage format (FP64 plain,
 FP32, FP16 and BF16 shifted populations with FP32 arithmetic) runs in its own
 context; the headline value is the FP16-stored path, the others are reported
-beside it.
+beside it.  Before the warm-up each context runs the library's launch-shape
+autotuner on its live arrays (``--no-autotune`` keeps the per-format
+defaults); the chosen shape is reported under ``launch``.
 
 MLUPS = cells * steps / wall seconds between two device synchronisations,
 the reference's formula (src/main.cpp:77-78) with the conserved-quantity
@@ -71,7 +73,7 @@ def run_mode(capi, prec, n, re, u0, steps, warmup, autotune, use_graph, mode="wr
         "roofline_frac": round(achieved / HBM_PEAK_GBS, 4),
         "roofline_mlups": round(HBM_PEAK_GBS * 1e9 / bpu / 1e6, 1),
         "launch": {k: info[k] for k in ("block", "cells_per_thread", "persistent", "nontemporal",
-                                        "dir_pad", "use_graph", "xcd_groups")},
+                                        "dir_pad", "use_graph", "xcd_groups", "xcd_run", "io16")},
         "autotune_best_ms": None if tuned_ms is None else round(tuned_ms, 4),
         "mass": cons[0], "ke": cons[4],
     }
@@ -125,8 +127,8 @@ def main():
     ap.add_argument("--u0", type=float, default=0.05)
     ap.add_argument("--modes", default="fp64,fp32,fp16,bf16")
     ap.add_argument("--headline", default="fp16")
-    ap.add_argument("--autotune", action="store_true",
-                    help="time launch shapes first and keep the fastest")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="skip the launch-shape autotuner (per-format defaults)")
     ap.add_argument("--graph", action="store_true", help="replay steps from a hipGraph")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--mode", choices=("wrap", "tiled"), default="wrap",
@@ -152,7 +154,7 @@ def main():
     for m in modes:
         print(f"bench: {m} {args.n}^3 steps={args.steps} warmup={args.warmup}", file=sys.stderr, flush=True)
         results[m] = run_mode(capi, m, args.n, args.re, args.u0, args.steps, args.warmup,
-                              args.autotune, args.graph, args.mode, args.tiles, args.halo)
+                              not args.no_autotune, args.graph, args.mode, args.tiles, args.halo)
         print(f"bench: {m} {results[m]['mlups']} MLUPS, {results[m]['achieved_gbs']} GB/s",
               file=sys.stderr, flush=True)
 

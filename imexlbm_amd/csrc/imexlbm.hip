@@ -415,7 +415,7 @@ int launch_step_t(imex_ctx *c, int op, int src, int dst, int t0, int n)
     const bool nt = c->nt != 0;
     // two cells per thread (tiles of 2*block cells or more) use the paired kernel
     const bool pair = c->cpt >= 2 && op == OP_FUSED;
-    const bool g16 = sizeof(S) == 2 && c->io16 == 1;
+    const bool g16 = c->io16 == 1;
 #define LAUNCH(W, O, N, P, G) hipLaunchKernelGGL((k_step<S, W, O, N, P, G>), g, b, 0, c->stream, sp, dp, c->d_boxes, tl, n, c->xcds, c->xcd_run)
 #define FUSED_G(W, G)                                                                                  \
     do {                                                                                               \
@@ -424,8 +424,7 @@ int launch_step_t(imex_ctx *c, int op, int src, int dst, int t0, int n)
     } while (0)
 #define FUSED(W)                                                 \
     do {                                                         \
-        if constexpr (sizeof(S) == 2) { if (g16) FUSED_G(W, true); else FUSED_G(W, false); } \
-        else FUSED_G(W, false);                                  \
+        if (g16) FUSED_G(W, true); else FUSED_G(W, false);       \
     } while (0)
     if (!c->tiled) {
         if (op == OP_FUSED) FUSED(true);
@@ -801,7 +800,7 @@ int imex_create(imex_ctx **out, const imex_config *cfg_in, const imex_domain *do
     // plain dispatch order with global 16-bit loads and stores.
     c->xcds = cfg.xcd_groups > 0 ? cfg.xcd_groups : (c->ssize == 2 ? 1 : 8);
     c->xcd_run = cfg.xcd_run >= 0 ? cfg.xcd_run : (c->ssize == 4 ? 64 : 0);
-    c->io16 = cfg.io16 >= 0 ? (cfg.io16 ? 1 : 0) : 1;
+    c->io16 = cfg.io16 >= 0 ? (cfg.io16 ? 1 : 0) : (c->ssize == 2 ? 1 : 0);
     auto bail = [&](int r) { imex_destroy(c); return r; };
 
     if (hipSetDevice(c->device) != hipSuccess) return bail(fail(IMEX_EHIP, "hipSetDevice failed"));
@@ -1089,52 +1088,61 @@ int imex_autotune(imex_ctx *c, int reps, double *best_ms)
     TRY(flush_pending(c));
     TRY(ensure_ghosts(c));
     HIPC(hipStreamSynchronize(c->stream));
-    struct Cand { int block, cpt, pers, nt, xcds, io16, run; };
+    // Candidates: workgroup size, cells per thread (2 = paired kernel), tile
+    // order over the XCDs (dispatch order, one range per XCD, runs of 4 or 64
+    // tiles) and the plane access path.  Streaming stores and one tile per
+    // workgroup are kept: the other settings never won in sweeps.
+    struct Cand { int block, cpt, xcds, run, io16; };
     std::vector<Cand> cands;
-    const int pers_opts[2] = {0, c->ncu * 8};
-    const int io_opts = c->ssize == 2 ? 2 : 1;
-    for (int blk : {128, 256, 512})
-        for (int cpt : {1, 2, 4})
-            for (int pers : pers_opts)
-                for (int nt : {0, 1})
-                    for (int x : {1, 8})
-                        for (int io = 0; io < io_opts; ++io)
-                            for (int run : {0, 4, 64}) {
-                                if (x == 1 && run) continue;
-                                cands.push_back({blk, cpt, pers, nt, x, io, run});
-                            }
-    const Cand orig = {c->block, c->cpt, c->persistent, c->nt, c->xcds, c->io16, c->xcd_run};
-    Cand best = orig;
-    double best_t = 1e300;
-    for (const Cand &k : cands) {
-        c->block = k.block; c->cpt = k.cpt; c->persistent = k.pers; c->nt = k.nt; c->xcds = k.xcds;
-        c->io16 = k.io16;
-        c->xcd_run = k.run;
+    for (int blk : {256, 512})
+        for (int cpt : {1, 2})
+            for (int io : {0, 1})
+                for (int order = 0; order < 4; ++order) {
+                    const int x = order == 0 ? 1 : 8;
+                    const int run = order <= 1 ? 0 : (order == 2 ? 4 : 64);
+                    cands.push_back({blk, cpt, x, run, io});
+                }
+    c->persistent = 0;
+    c->nt = 1;
+    auto apply = [&](const Cand &k) -> int {
+        c->block = k.block; c->cpt = k.cpt; c->xcds = k.xcds; c->xcd_run = k.run; c->io16 = k.io16;
         build_tiles(c);
-        TRY(upload_tiles(c));
-        // warm-up, then timed: fused launches cur -> other without swapping,
-        // so the state is untouched
+        return upload_tiles(c);
+    };
+    // fused launches cur -> other without swapping: the state is untouched
+    auto time_it = [&](int n, double &ms_out) -> int {
         TRY(launch_step(c, OP_FUSED, c->cur, 1 - c->cur));
         HIPC(hipEventRecord(c->ev[0], c->stream));
-        for (int r = 0; r < reps; ++r) TRY(launch_step(c, OP_FUSED, c->cur, 1 - c->cur));
+        for (int r = 0; r < n; ++r) TRY(launch_step(c, OP_FUSED, c->cur, 1 - c->cur));
         HIPC(hipEventRecord(c->ev[1], c->stream));
         HIPC(hipEventSynchronize(c->ev[1]));
         float ms = 0.f;
         HIPC(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
-        const double t = ms / reps;
+        ms_out = ms / n;
+        return IMEX_OK;
+    };
+    std::vector<std::pair<double, int>> t(cands.size());
+    for (size_t k = 0; k < cands.size(); ++k) {
+        TRY(apply(cands[k]));
+        TRY(time_it(reps, t[k].first));
+        t[k].second = (int)k;
         if (std::getenv("IMEXLBM_AUTOTUNE_LOG"))
-            std::fprintf(stderr, "autotune block=%d cpt=%d persistent=%d nt=%d xcds=%d run=%d io16=%d: %.4f ms\n",
-                         k.block, k.cpt, k.pers, k.nt, k.xcds, k.run, k.io16, t);
-        if (t < best_t) { best_t = t; best = k; }
+            std::fprintf(stderr, "autotune block=%d cpt=%d xcds=%d run=%d io16=%d: %.4f ms\n", cands[k].block,
+                         cands[k].cpt, cands[k].xcds, cands[k].run, cands[k].io16, t[k].first);
     }
-    c->block = best.block; c->cpt = best.cpt; c->persistent = best.pers; c->nt = best.nt;
-    c->xcds = best.xcds;
-    c->io16 = best.io16;
-    c->xcd_run = best.run;
-    build_tiles(c);
-    TRY(upload_tiles(c));
+    // confirmation round: the four fastest again, best of both rounds
+    std::sort(t.begin(), t.end());
+    const size_t top = std::min<size_t>(4, t.size());
+    for (size_t k = 0; k < top; ++k) {
+        TRY(apply(cands[t[k].second]));
+        double again = 0.0;
+        TRY(time_it(reps, again));
+        t[k].first = std::min(t[k].first, again);
+    }
+    std::sort(t.begin(), t.begin() + top);
+    TRY(apply(cands[t[0].second]));
     drop_graphs(c);
-    if (best_ms) *best_ms = best_t;
+    if (best_ms) *best_ms = t[0].first;
     return IMEX_OK;
 }
 
@@ -1152,16 +1160,24 @@ int imex_copy_probe(int device, long long bytes, int reps, double *gbs)
     hipStream_t s = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int r = IMEX_OK;
-    const int grid = (int)((n + 1023) / 1024);
     if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
         hipEventCreate(&e1) != hipSuccess)
         r = fail(IMEX_EHIP, "copy probe stream/event creation failed");
-    // best of plain and streaming stores
+    // best of four variants: one or four 16-byte loads per lane (one 4 KiB or
+    // 16 KiB run per workgroup), plain or streaming stores
     double best = 0.0;
-    for (int nt = 0; nt < 2 && r == IMEX_OK; ++nt) {
+    for (int v = 0; v < 4 && r == IMEX_OK; ++v) {
+        const int u = v < 2 ? 1 : 4;
+        const long long grid = (n + 256LL * u - 1) / (256LL * u);
         auto launch = [&]() {
-            if (nt) hipLaunchKernelGGL(k_copy<true>, dim3(grid), dim3(256), 0, s, (const u32x4 *)a, (u32x4 *)b, n);
-            else hipLaunchKernelGGL(k_copy<false>, dim3(grid), dim3(256), 0, s, (const u32x4 *)a, (u32x4 *)b, n);
+            const u32x4 *sa = (const u32x4 *)a;
+            u32x4 *db = (u32x4 *)b;
+            switch (v) {
+            case 0: hipLaunchKernelGGL((k_copy<false, 1>), dim3(grid), dim3(256), 0, s, sa, db, n); break;
+            case 1: hipLaunchKernelGGL((k_copy<true, 1>), dim3(grid), dim3(256), 0, s, sa, db, n); break;
+            case 2: hipLaunchKernelGGL((k_copy<false, 4>), dim3(grid), dim3(256), 0, s, sa, db, n); break;
+            default: hipLaunchKernelGGL((k_copy<true, 4>), dim3(grid), dim3(256), 0, s, sa, db, n); break;
+            }
         };
         launch();
         (void)hipEventRecord(e0, s);

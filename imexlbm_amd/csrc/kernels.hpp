@@ -199,10 +199,10 @@ template <> __device__ inline unsigned short narrow16<bf16_t>(float v)
 // z-planes z-1, z, z+1 and of destination z-plane z.  Offsets are bytes: the
 // lane's in-plane cell (vo) plus the direction sub-plane (so, wavefront
 // uniform).  4- and 8-byte formats use buffer descriptors, which take the
-// direction offset as a scalar operand.  2-byte formats can instead use global
+// direction offset as a scalar operand.  Any format can instead use global
 // loads and stores with a uniform z-plane base and one 32-bit lane offset
-// (the io16 launch knob): which of the two streams 16-bit data faster is
-// measured, not assumed.
+// (the io16 launch knob, "global IO"): which of the two streams faster is
+// measured per format, not assumed.
 template <class S, bool GLOBAL> struct Planes;
 
 template <class S> struct Planes<S, false> {
@@ -223,23 +223,34 @@ template <class S> struct Planes<S, false> {
 };
 
 template <class S> struct Planes<S, true> {
-    using R = float;
+    using R = typename Store<S>::real;
     const char *m, *c, *p;
     char *w;
     __device__ Planes(const S *zm, const S *z0, const S *zp, S *zw)
         : m(reinterpret_cast<const char *>(zm)), c(reinterpret_cast<const char *>(z0)),
           p(reinterpret_cast<const char *>(zp)), w(reinterpret_cast<char *>(zw)) {}
+    template <int EZV> __device__ const char *src() const { return EZV == 0 ? c : (EZV == 1 ? m : p); }
     template <int EZV> __device__ unsigned short raw(int vo, int so) const
     {
-        const char *b = EZV == 0 ? c : (EZV == 1 ? m : p);
-        return *reinterpret_cast<const unsigned short *>(b + (unsigned long long)(unsigned)(so + vo));
+        return *reinterpret_cast<const unsigned short *>(src<EZV>() + (unsigned long long)(unsigned)(so + vo));
     }
-    template <int EZV> __device__ float ld(int vo, int so) const { return widen16<S>(raw<EZV>(vo, so)); }
-    template <bool NT> __device__ void st(float v, int vo, int so) const
+    template <int EZV> __device__ R ld(int vo, int so) const
     {
-        unsigned short *q = reinterpret_cast<unsigned short *>(w + (unsigned long long)(unsigned)(so + vo));
-        const unsigned short b = narrow16<S>(v);
-        if constexpr (NT) __builtin_nontemporal_store(b, q); else *q = b;
+        const char *q = src<EZV>() + (unsigned long long)(unsigned)(so + vo);
+        if constexpr (sizeof(S) == 2) return widen16<S>(*reinterpret_cast<const unsigned short *>(q));
+        else return *reinterpret_cast<const S *>(q);
+    }
+    template <bool NT> __device__ void st(R v, int vo, int so) const
+    {
+        char *q = w + (unsigned long long)(unsigned)(so + vo);
+        if constexpr (sizeof(S) == 2) {
+            const unsigned short b = narrow16<S>(v);
+            if constexpr (NT) __builtin_nontemporal_store(b, reinterpret_cast<unsigned short *>(q));
+            else *reinterpret_cast<unsigned short *>(q) = b;
+        } else {
+            if constexpr (NT) __builtin_nontemporal_store((S)v, reinterpret_cast<S *>(q));
+            else *reinterpret_cast<S *>(q) = (S)v;
+        }
     }
 };
 
@@ -321,7 +332,7 @@ k_step(const S *__restrict__ src, S *__restrict__ dst, const BoxDesc *__restrict
             if (zp == d.nz) zp = 0;
         }
         const S *sbox = src + d.base + d.zorg;
-        const Planes<S, G16 && sizeof(S) == 2> P(sbox + (long long)zm * d.sz, sbox + (long long)z * d.sz, sbox + (long long)zp * d.sz,
+        const Planes<S, G16> P(sbox + (long long)zm * d.sz, sbox + (long long)z * d.sz, sbox + (long long)zp * d.sz,
                           dst + d.base + d.zorg + (long long)z * d.sz);
         const int dsb = (int)(d.dstride * (long long)sizeof(S));
         if constexpr (PAIR) {
@@ -649,22 +660,23 @@ k_halo(const S *__restrict__ src, S *__restrict__ dst, const HaloSeg *__restrict
 }
 
 // ---------------------------------------------------------------------------
-// Plain device copy, 16 bytes per lane, four independent loads in flight per
-// lane: the measured HBM ceiling that the step kernels are compared with.
+// Plain device copy, 16 bytes per lane, U loads in flight per lane (each
+// workgroup copies one run of U * 4 KiB): the measured HBM ceiling that the
+// step kernels are compared with.
 // ---------------------------------------------------------------------------
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
-template <bool NT>
+template <bool NT, int U>
 __global__ void __launch_bounds__(256)
 k_copy(const u32x4 *__restrict__ src, u32x4 *__restrict__ dst, long long n)
 {
-    const long long base = (long long)blockIdx.x * 1024 + threadIdx.x;
-    u32x4 v[4];
+    const long long base = (long long)blockIdx.x * (256 * U) + threadIdx.x;
+    u32x4 v[U];
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int u = 0; u < U; ++u)
         if (base + u * 256 < n) v[u] = src[base + u * 256];
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int u = 0; u < U; ++u)
         if (base + u * 256 < n) {
             if constexpr (NT) __builtin_nontemporal_store(v[u], dst + base + u * 256);
             else dst[base + u * 256] = v[u];

@@ -166,9 +166,11 @@ int imex_synchronize(imex_ctx *ctx);
 int imex_timed_steps(imex_ctx *ctx, int nsteps, double *wall_ms,
                      double *kernel_ms);
 
-/* Time candidate launch shapes (block, cells per thread, persistent grid,
- * streaming stores) on the live arrays without changing the state, and keep
- * the fastest.  Blocks.  best_ms (may be NULL) receives its step time. */
+/* Time candidate launch shapes (workgroup size, cells per thread, tile order
+ * over the XCDs, plane access path) on the live arrays without changing the
+ * state, confirm the four fastest with a second timing, and keep the best.
+ * Streaming stores and one work tile per workgroup are fixed.  Blocks.
+ * best_ms (may be NULL) receives its step time. */
 int imex_autotune(imex_ctx *ctx, int reps, double *best_ms);
 
 int imex_query(imex_ctx *ctx, imex_info *info);
@@ -184,8 +186,9 @@ int imex_profile_reset(imex_ctx *ctx);
 int imex_profile_read(imex_ctx *ctx, int phase, double out[4], long long *calls);
 
 /* Measured HBM ceiling: a plain device-to-device copy of `bytes` (16 bytes
- * per lane, streaming stores), timed over `reps` launches; *gbs receives read
- * plus write bytes per second / 1e9.  No counterpart in the reference, whose
+ * per lane; one or four loads per lane, plain or streaming stores: the best
+ * of the four), timed over `reps` launches; *gbs receives read plus write
+ * bytes per second / 1e9.  No counterpart in the reference, whose
  * phase profiler assumes a fixed 1600 GB/s peak instead
  * (src/profile.hpp:67-73); the step kernels are reported against this. */
 int imex_copy_probe(int device, long long bytes, int reps, double *gbs);
