@@ -13,10 +13,13 @@ defaults); the chosen shape is reported under ``launch``.
 
 MLUPS = cells * steps / wall seconds between two device synchronisations,
 the reference's formula (src/main.cpp:77-78) with the conserved-quantity
-reduction outside the timed region.  The roofline figure divides the
-algorithmic bytes of one step (2 * 27 * sizeof(storage) per cell) by the mean
-duration of the step kernel, measured with HIP events on the stream the
-kernel runs on, against the 8 TB/s HBM peak.
+reduction outside the timed region.  Where the domain allows it the library
+advances two steps per pass (temporal blocking, bit-identical to two single
+steps).  The roofline figure divides the algorithmic bytes of the updates one
+launch performs (2 * 27 * sizeof(storage) per cell per update) by the mean
+duration of the launch, measured with HIP events on the stream the kernel
+runs on, against the 8 TB/s HBM peak; ``hbm_gbs`` is the measured HBM traffic
+per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE) over the same duration.
 
 The CPU baseline is the FP64 oracle (the reference's collide / 26-slab ghost
 copy / pull-stream structure in C with OpenMP) timed on this host's cores on
@@ -73,7 +76,9 @@ def run_mode(capi, prec, n, re, u0, steps, warmup, autotune, use_graph, mode="wr
         "roofline_frac": round(achieved / HBM_PEAK_GBS, 4),
         "roofline_mlups": round(HBM_PEAK_GBS * 1e9 / bpu / 1e6, 1),
         "launch": {k: info[k] for k in ("block", "cells_per_thread", "persistent", "nontemporal",
-                                        "dir_pad", "use_graph", "xcd_groups", "xcd_run", "io16")},
+                                        "dir_pad", "use_graph", "xcd_groups", "xcd_run", "io16",
+                                        "two_step", "two_step_tile", "two_step_zseg", "two_step_xcds")},
+        "updates_per_launch": 2 if info["two_step"] else 1,
         "autotune_best_ms": None if tuned_ms is None else round(tuned_ms, 4),
         "mass": cons[0], "ke": cons[4],
     }
@@ -106,15 +111,43 @@ def cpu_baseline(target_s=12.0):
                       f"{t3 - t2:.1f} s"}
 
 
-def traffic_from_profiles(prec, n):
-    """HBM bytes per launch of the step kernel from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary, if one exists for this workload."""
+def traffic_from_profiles(prec, n, updates):
+    """HBM bytes per launch of the step kernel (one step, or two steps per
+    pass) from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary, if one
+    exists for this workload."""
     path = os.path.join(ROOT, "profiles", f"pmc_fused_step_tgv{n}.md")
     if not os.path.exists(path):
         return None
-    pat = re.compile(r"^\|\s*" + re.escape(prec) + r"\s*\|.*?\|\s*traffic_bytes=(\d+)", re.M)
+    key = f"traffic{updates}_bytes="
+    pat = re.compile(r"^\|\s*" + re.escape(prec) + r"\s*\|.*?\|\s*" + re.escape(key) + r"(\d+)", re.M)
     m = pat.search(open(path).read())
     return int(m.group(1)) if m else None
+
+
+def roofline(h, traffic, copy_gbs):
+    """Roofline of the dominant kernel.  achieved = algorithmic bytes per
+    lattice update (2 * 27 * sizeof) x updates per launch / mean launch time.
+    With two steps per pass the kernel reads and writes each population once
+    per two updates, so HBM moves about half the algorithmic bytes: hbm_gbs
+    (measured traffic / launch time) is what the memory system delivers."""
+    ups = h["updates_per_launch"]
+    launch_s = h["kernel_ms_per_step"] * ups / 1e3
+    out = {
+        "bound": "hbm",
+        "achieved": h["achieved_gbs"],
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": h["roofline_frac"],
+        "traffic": traffic,
+        "updates_per_launch": ups,
+        "launch_ms": round(launch_s * 1e3, 4),
+        "copy_ceiling": round(copy_gbs, 1),
+        "frac_of_copy": round(h["achieved_gbs"] / copy_gbs, 4),
+    }
+    if traffic:
+        out["hbm_gbs"] = round(traffic / launch_s / 1e9, 1)
+        out["hbm_frac"] = round(traffic / launch_s / 1e9 / HBM_PEAK_GBS, 4)
+    return out
 
 
 def main():
@@ -190,16 +223,8 @@ def main():
             "fp16_over_fp64": (round(results["fp16"]["mlups"] / results["fp64"]["mlups"], 3)
                                if "fp16" in results and "fp64" in results else None),
         },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": h["achieved_gbs"],
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": h["roofline_frac"],
-            "traffic": traffic_from_profiles(args.headline, args.n),
-            "copy_ceiling": round(copy_gbs, 1),
-            "frac_of_copy": round(h["achieved_gbs"] / copy_gbs, 4),
-        },
+        "roofline": roofline(h, traffic_from_profiles(args.headline, args.n, h["updates_per_launch"]),
+                             copy_gbs),
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -38,7 +38,8 @@ class Config(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int) for n in (
         "precision", "mode", "ntiles", "halo_depth", "block", "cells_per_thread",
         "persistent", "nontemporal", "dir_pad", "use_graph", "sequential", "device",
-        "xcd_groups", "contiguous", "io16", "profile", "xcd_run")]
+        "xcd_groups", "contiguous", "io16", "profile", "xcd_run", "two_step", "two_step_tile",
+        "two_step_zseg", "two_step_xcds", "two_step_prefetch")]
 
 
 class Info(ctypes.Structure):
@@ -46,7 +47,8 @@ class Info(ctypes.Structure):
                 ("device_bytes", ctypes.c_longlong)] + [(n, ctypes.c_int) for n in (
                     "storage_bytes", "ndomains", "ntiles", "work_tiles", "block",
                     "cells_per_thread", "persistent", "nontemporal", "dir_pad", "use_graph",
-                    "grid", "rx", "ry", "rz", "xcd_groups", "contiguous", "io16", "xcd_run")]
+                    "grid", "rx", "ry", "rz", "xcd_groups", "contiguous", "io16", "xcd_run",
+                    "two_step", "two_step_tile", "two_step_zseg", "two_step_xcds", "two_step_prefetch")]
 
 
 class ImexError(RuntimeError):

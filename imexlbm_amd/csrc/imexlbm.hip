@@ -98,10 +98,10 @@ int block_lo(int gl, int r, int p) { return p * (gl / r) + std::min(p, gl % r); 
 
 // Phases of the per-phase profiler (the counterpart of the reference's
 // LBM_PROFILE build, src/profile.hpp:42-113).
-enum Phase { PH_COLLIDE = 0, PH_FUSED, PH_STREAM, PH_HALO, PH_CONSERVED, PH_MACRO, PH_N };
+enum Phase { PH_COLLIDE = 0, PH_FUSED, PH_STREAM, PH_HALO, PH_CONSERVED, PH_MACRO, PH_FUSED2, PH_N };
 const char *const kPhaseNames[PH_N] = {"collide in place (first step)", "fused pull-stream + collide",
                                        "bare stream", "ghost fill (pack + unpack)", "conserved sums",
-                                       "macroscopic fields"};
+                                       "macroscopic fields", "two fused steps in one pass"};
 struct PhaseStat {
     double ms = 0, mn = 0, mx = 0, bytes = 0;
     long long calls = 0;
@@ -153,8 +153,12 @@ struct imex_ctx {
     int contiguous = 0;
     int io16 = 0;
     int xcd_run = 0;
+    // two steps per pass: on/off, column tile, z-planes per sweep, tile order
+    int two_step = 0, ts_tile = 0, ts_zseg = 1, ts_xcds = 1, ts_pf = 0;
+    int deferred = 0;            // fused steps owed by Streaming() calls, < 2
     hipGraph_t graph[2] = {nullptr, nullptr};
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    bool gflip[2] = {false, false};   // replaying graph[p] leaves the state in pool[1-p]
 
     long long cells = 0;
     long long device_bytes = 0;
@@ -462,6 +466,93 @@ int launch_step(imex_ctx *c, int op, int src, int dst)
     return one(0, (int)c->tiles.size());
 }
 
+// Two-step pass: column tile {TX, TY} of each tile variant (3 is the 32x8
+// tile with 384 threads: one ring cell per thread).
+void ts_dims(int tile, int &tx, int &ty)
+{
+    tx = tile == 0 ? 64 : 32;
+    ty = tile == 1 ? 16 : 8;
+}
+
+// One periodic domain in wrap mode whose x and y extents the column tile
+// divides; FP64 builds only the 32x8 tiles (its LDS ring needs 147 KB).
+bool ts_allowed(const imex_ctx *c, int tile)
+{
+    if (c->tiled || c->inst.size() != 1 || c->boxes.size() != 1 || tile < 0 || tile > 3) return false;
+    if (c->ssize == 8 && tile < 2) return false;
+    int tx, ty;
+    ts_dims(tile, tx, ty);
+    return c->boxes[0].nx % tx == 0 && c->boxes[0].ny % ty == 0;
+}
+
+// Enough z-segments that the pass has about 8 workgroups per CU (2-byte
+// formats, two 71 KB workgroups fit a CU) or one per CU (others), and at
+// least 32 planes per sweep where the box has them: each sweep recomputes
+// two intermediate planes.
+int ts_default_zseg(const imex_ctx *c, int tile)
+{
+    int tx, ty;
+    ts_dims(tile, tx, ty);
+    const BoxDesc &b = c->boxes[0];
+    const long long ntile = (long long)(b.nx / tx) * (b.ny / ty);
+    const long long want = (c->ssize == 2 ? 8LL : 1LL) * c->ncu;
+    const long long nseg = std::max(1LL, (want + ntile - 1) / ntile);
+    const int zseg = (int)((b.nz + nseg - 1) / nseg);
+    return std::max(zseg, std::min(32, b.nz));
+}
+
+template <class S, int TX, int TY, int NTH>
+int launch_ts(imex_ctx *c, int src, int dst)
+{
+    const BoxDesc &b = c->boxes[0];
+    const int zseg = std::max(1, std::min(c->ts_zseg, b.nz));
+    const long long grid = (long long)(b.nx / TX) * (b.ny / TY) * ((b.nz + zseg - 1) / zseg);
+    if (grid > 0x7fffffffLL) return fail(IMEX_ERANGE, "two-step grid too large");
+    const S *sp = (const S *)pool_ptr(c, src);
+    S *dp = (S *)pool_ptr(c, dst);
+    if constexpr (sizeof(S) >= 4) {
+        if (c->ts_pf) {
+            hipLaunchKernelGGL((k_step2<S, TX, TY, NTH, true, true>), dim3((unsigned)grid), dim3(NTH), 0, c->stream,
+                               sp, dp, c->d_boxes, c->ts_xcds, zseg);
+            HIPC(hipGetLastError());
+            return IMEX_OK;
+        }
+    }
+    hipLaunchKernelGGL((k_step2<S, TX, TY, NTH, true, false>), dim3((unsigned)grid), dim3(NTH), 0, c->stream,
+                       sp, dp, c->d_boxes, c->ts_xcds, zseg);
+    HIPC(hipGetLastError());
+    return IMEX_OK;
+}
+
+template <class S>
+int launch_step2_t(imex_ctx *c, int src, int dst)
+{
+    if constexpr (sizeof(S) == 8) {
+        if (c->ts_tile == 3) return launch_ts<S, 32, 8, 384>(c, src, dst);
+        return launch_ts<S, 32, 8, 256>(c, src, dst);
+    } else {
+        switch (c->ts_tile) {
+        case 0: return launch_ts<S, 64, 8, 512>(c, src, dst);
+        case 1: return launch_ts<S, 32, 16, 512>(c, src, dst);
+        case 3: return launch_ts<S, 32, 8, 384>(c, src, dst);
+        default: return launch_ts<S, 32, 8, 256>(c, src, dst);
+        }
+    }
+}
+
+// Two fused steps pool[src] -> pool[dst] in one pass (the caller checked
+// ts_allowed for c->ts_tile).
+int launch_step2(imex_ctx *c, int src, int dst)
+{
+    PhaseScope ps(c, PH_FUSED2, 2.0 * 2.0 * Q * c->ssize * (double)c->cells);
+    switch (c->prec) {
+    case IMEX_F64: return launch_step2_t<double>(c, src, dst);
+    case IMEX_F32: return launch_step2_t<float>(c, src, dst);
+    case IMEX_F16: return launch_step2_t<_Float16>(c, src, dst);
+    default: return launch_step2_t<bf16_t>(c, src, dst);
+    }
+}
+
 template <class S>
 int launch_halo_t(imex_ctx *c, int which)
 {
@@ -504,6 +595,32 @@ int do_fused(imex_ctx *c)
     return IMEX_OK;
 }
 
+int do_fused2(imex_ctx *c)
+{
+    TRY(launch_step2(c, c->cur, 1 - c->cur));
+    c->cur = 1 - c->cur;
+    c->ghosts_valid = false;
+    return IMEX_OK;
+}
+
+// Two fused steps: one pass when the two-step mode is on.
+int do_pair(imex_ctx *c)
+{
+    if (c->two_step) return do_fused2(c);
+    TRY(do_fused(c));
+    return do_fused(c);
+}
+
+// Launch a fused step that a Streaming() call left owed.
+int flush_deferred(imex_ctx *c)
+{
+    if (c->deferred > 0) {
+        c->deferred = 0;
+        TRY(do_fused(c));
+    }
+    return IMEX_OK;
+}
+
 int do_stream(imex_ctx *c)
 {
     TRY(ensure_ghosts(c));
@@ -517,6 +634,7 @@ int do_stream(imex_ctx *c)
 // state becomes the collided one, stored as S.
 int flush_pending(imex_ctx *c)
 {
+    TRY(flush_deferred(c));
     if (!c->pending) return IMEX_OK;
     if (c->state_S) TRY(do_collide(c));
     else TRY(do_fused(c));
@@ -553,9 +671,9 @@ int build_graph(imex_ctx *c, int parity)
     c->ghosts_valid = false;
     HIPC(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     c->capturing = true;
-    int r = do_fused(c);
-    if (r == IMEX_OK) r = do_fused(c);
+    const int r = do_pair(c);
     c->capturing = false;
+    c->gflip[parity] = c->cur != parity;   // one two-step pass flips the pools, two steps do not
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(c->stream, &g);
     c->cur = save_cur;
@@ -572,11 +690,19 @@ int build_graph(imex_ctx *c, int parity)
 
 int run_steps(imex_ctx *c, int nsteps)
 {
+    if (nsteps > 0) TRY(flush_deferred(c));
     while (nsteps > 0) {
-        if (c->cfg.use_graph && !c->profile && nsteps >= 2 && !c->state_S && !c->pending) {
-            if (!c->gexec[c->cur]) TRY(build_graph(c, c->cur));
-            HIPC(hipGraphLaunch(c->gexec[c->cur], c->stream));
-            c->ghosts_valid = false;
+        if (nsteps >= 2 && !c->state_S && !c->pending) {
+            if (c->cfg.use_graph && !c->profile) {
+                if (!c->gexec[c->cur]) TRY(build_graph(c, c->cur));
+                HIPC(hipGraphLaunch(c->gexec[c->cur], c->stream));
+                if (c->gflip[c->cur]) c->cur = 1 - c->cur;
+                c->ghosts_valid = false;
+            } else if (c->two_step) {
+                TRY(do_fused2(c));
+            } else {
+                TRY(do_pair(c));
+            }
             nsteps -= 2;
             continue;
         }
@@ -713,6 +839,9 @@ void imex_config_default(imex_config *cfg)
     cfg->nontemporal = -1;
     cfg->xcd_run = -1;
     cfg->io16 = -1;
+    cfg->two_step = -1;
+    cfg->two_step_tile = -1;
+    cfg->two_step_prefetch = -1;
 }
 
 const char *imex_strerror(int s)
@@ -829,6 +958,21 @@ int imex_create(imex_ctx **out, const imex_config *cfg_in, const imex_domain *do
     int r = layout_boxes(c);
     if (r != IMEX_OK) return bail(r);
     build_tiles(c);
+    {
+        const int def_tile = c->ssize == 2 ? 0 : (c->ssize == 4 ? 1 : 2);
+        c->ts_tile = cfg.two_step_tile >= 0 ? cfg.two_step_tile : def_tile;
+        if (!ts_allowed(c, c->ts_tile) && cfg.two_step_tile < 0)
+            for (int t : {0, 1, 2})
+                if (ts_allowed(c, t)) { c->ts_tile = t; break; }
+        const bool ok = ts_allowed(c, c->ts_tile);
+        if (cfg.two_step == 1 && !ok)
+            return bail(fail(IMEX_EINVAL, "two_step needs one periodic domain (IMEX_WRAP) whose nx and ny the "
+                                          "column tile divides (FP64: 32x8 only)"));
+        c->two_step = (cfg.two_step != 0 && ok) ? 1 : 0;
+        c->ts_zseg = cfg.two_step_zseg > 0 ? cfg.two_step_zseg : (ok ? ts_default_zseg(c, c->ts_tile) : 1);
+        c->ts_xcds = cfg.two_step_xcds > 0 ? cfg.two_step_xcds : (c->ssize == 2 ? 1 : 8);
+        c->ts_pf = c->ssize >= 4 && cfg.two_step_prefetch != 0 ? 1 : 0;
+    }
     if (c->tiled) build_halo(c);
 
     const size_t pool_bytes = (size_t)c->pool_elems * c->ssize;
@@ -896,6 +1040,7 @@ int imex_initialize(imex_ctx *c)
     c->initialized = true;
     c->state_S = true;
     c->pending = false;
+    c->deferred = 0;
     c->ghosts_valid = false;
     HIPC(hipStreamSynchronize(c->stream));
     return IMEX_OK;
@@ -905,7 +1050,7 @@ int imex_collision(imex_ctx *c)
 {
     TRY(check_ctx(c));
     if (!c->initialized) return fail(IMEX_ESTATE, "imex_initialize has not been called");
-    TRY(flush_pending(c));
+    if (c->pending) TRY(flush_pending(c));
     c->pending = true;
     return IMEX_OK;
 }
@@ -916,9 +1061,20 @@ int imex_streaming(imex_ctx *c)
     if (!c->initialized) return fail(IMEX_ESTATE, "imex_initialize has not been called");
     if (c->pending) {
         c->pending = false;
-        if (c->state_S) TRY(do_collide(c));
-        else TRY(do_fused(c));
+        if (c->state_S) {
+            TRY(do_collide(c));
+        } else if (c->two_step && !c->profile) {
+            // fused steps go out in pairs, one pass each; an odd one left
+            // over is launched by the next call that needs the state
+            if (++c->deferred == 2) {
+                c->deferred = 0;
+                TRY(do_fused2(c));
+            }
+        } else {
+            TRY(do_fused(c));
+        }
     } else if (!c->state_S) {
+        TRY(flush_deferred(c));
         TRY(do_stream(c));
     }
     c->state_S = false;
@@ -1057,6 +1213,7 @@ int imex_set_f(imex_ctx *c, int domain, const double *host)
 int imex_synchronize(imex_ctx *c)
 {
     TRY(check_ctx(c));
+    TRY(flush_deferred(c));
     HIPC(hipStreamSynchronize(c->stream));
     return IMEX_OK;
 }
@@ -1141,8 +1298,56 @@ int imex_autotune(imex_ctx *c, int reps, double *best_ms)
     }
     std::sort(t.begin(), t.begin() + top);
     TRY(apply(cands[t[0].second]));
+    double best = t[0].first;
+
+    // Two steps per pass: column tile x z-planes per sweep x tile order,
+    // timed per pass (two steps) against two single-step launches.
+    if (c->cfg.two_step != 0) {
+        struct Cand2 { int tile, zseg, xcds, pf; };
+        std::vector<Cand2> c2;
+        const int nz = c->boxes.empty() ? 0 : c->boxes[0].nz;
+        const int pf_max = (c->ssize >= 4 && c->cfg.two_step_prefetch != 0) ? 1 : 0;
+        const int pf_min = (c->cfg.two_step_prefetch == 1 && pf_max) ? 1 : 0;
+        for (int tile : {0, 1, 2, 3}) {
+            if (!ts_allowed(c, tile)) continue;
+            std::vector<int> zs = {ts_default_zseg(c, tile)};
+            for (int div : {1, 2, 4, 8, 16})
+                if (nz / div >= 16 && std::find(zs.begin(), zs.end(), nz / div) == zs.end()) zs.push_back(nz / div);
+            for (int z : zs)
+                for (int x : {1, 8})
+                    for (int pf = pf_min; pf <= pf_max; ++pf) c2.push_back({tile, z, x, pf});
+        }
+        const int keep[5] = {c->two_step, c->ts_tile, c->ts_zseg, c->ts_xcds, c->ts_pf};
+        double best2 = 1e300;
+        int pick = -1;
+        for (size_t k = 0; k < c2.size(); ++k) {
+            c->ts_tile = c2[k].tile; c->ts_zseg = c2[k].zseg; c->ts_xcds = c2[k].xcds; c->ts_pf = c2[k].pf;
+            TRY(launch_step2(c, c->cur, 1 - c->cur));
+            HIPC(hipEventRecord(c->ev[0], c->stream));
+            for (int r = 0; r < reps; ++r) TRY(launch_step2(c, c->cur, 1 - c->cur));
+            HIPC(hipEventRecord(c->ev[1], c->stream));
+            HIPC(hipEventSynchronize(c->ev[1]));
+            float ms = 0.f;
+            HIPC(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+            const double per_step = ms / reps / 2.0;
+            if (std::getenv("IMEXLBM_AUTOTUNE_LOG"))
+                std::fprintf(stderr, "autotune two-step tile=%d zseg=%d xcds=%d pf=%d: %.4f ms per step\n",
+                             c2[k].tile, c2[k].zseg, c2[k].xcds, c2[k].pf, per_step);
+            if (per_step < best2) { best2 = per_step; pick = (int)k; }
+        }
+        if (pick >= 0 && (best2 < best || c->cfg.two_step == 1)) {
+            // faster, or requested: keep it with its fastest shape
+            c->two_step = 1;
+            c->ts_tile = c2[pick].tile; c->ts_zseg = c2[pick].zseg; c->ts_xcds = c2[pick].xcds;
+            c->ts_pf = c2[pick].pf;
+            best = std::min(best, best2);
+        } else {
+            c->two_step = pick >= 0 ? 0 : keep[0];
+            c->ts_tile = keep[1]; c->ts_zseg = keep[2]; c->ts_xcds = keep[3]; c->ts_pf = keep[4];
+        }
+    }
     drop_graphs(c);
-    if (best_ms) *best_ms = t[0].first;
+    if (best_ms) *best_ms = best;
     return IMEX_OK;
 }
 
@@ -1244,6 +1449,11 @@ int imex_query(imex_ctx *c, imex_info *info)
     info->contiguous = c->contiguous;
     info->io16 = c->io16;
     info->xcd_run = c->xcd_run;
+    info->two_step = c->two_step;
+    info->two_step_tile = c->ts_tile;
+    info->two_step_zseg = c->ts_zseg;
+    info->two_step_xcds = c->ts_xcds;
+    info->two_step_prefetch = c->ts_pf;
     info->grid = c->tiled ? 1 : 0;
     info->rx = c->rgrid[0]; info->ry = c->rgrid[1]; info->rz = c->rgrid[2];
     return IMEX_OK;

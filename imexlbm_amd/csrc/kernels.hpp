@@ -390,6 +390,166 @@ k_step(const S *__restrict__ src, S *__restrict__ dst, const BoxDesc *__restrict
 }
 
 // ---------------------------------------------------------------------------
+// Two steps in one pass (temporal blocking) for one periodic box.
+//
+// A workgroup owns a column of TX x TY cells and sweeps z over [z0, z1).  For
+// every z-plane p it first computes the intermediate step A(p) = C(pull(P)) on
+// the column plus a one-cell ring (the cells the second step pulls from),
+// reading P from HBM, and keeps A(p) in LDS; then the output plane
+// B(k) = C(pull(A)) of the plane below, whose pull reads A(k-1), A(k), A(k+1)
+// from LDS, goes to HBM.  Each population is read from HBM once and written
+// once per two steps; the ring cells (a fraction (TX+2)(TY+2)/(TX*TY) - 1 of
+// the column) are collided twice.  A(p) is rounded to the storage format on
+// its way into LDS, exactly as a stored step would be, so the result is
+// bit-identical to two separate steps.
+//
+// LDS holds A by direction group: B(k) pulls the 9 directions with e_z = -1
+// from A(k+1), the 9 with e_z = 0 from A(k) and the 9 with e_z = +1 from
+// A(k-1).  So the e_z = -1 group needs one plane slot, e_z = 0 two and e_z = +1
+// three: 54 direction planes of (TX+2)(TY+2) cells instead of 81.
+// ---------------------------------------------------------------------------
+__host__ __device__ constexpr int ez_group(int ii) { return EZ[ii] + 1; }   // 0: -1, 1: 0, 2: +1
+__host__ __device__ constexpr int ez_index(int ii)
+{
+    int k = 0;
+    for (int j = 0; j < ii; ++j)
+        if (EZ[j] == EZ[ii]) ++k;
+    return k;
+}
+
+template <class S> struct LdsWord { using type = S; };
+template <> struct LdsWord<_Float16> { using type = unsigned short; };
+template <> struct LdsWord<bf16_t> { using type = unsigned short; };
+
+template <class S> __device__ inline typename LdsWord<S>::type to_word(typename Store<S>::real v)
+{
+    if constexpr (sizeof(S) == 2) return narrow16<S>(v);
+    else return (S)v;
+}
+template <class S> __device__ inline typename Store<S>::real from_word(typename LdsWord<S>::type v)
+{
+    if constexpr (sizeof(S) == 2) return widen16<S>(v);
+    else return v;
+}
+
+template <class S, int RC, int... I>
+__device__ inline void lds_store_cell(typename LdsWord<S>::type *gm, typename LdsWord<S>::type *g0,
+                                      typename LdsWord<S>::type *gp, int q,
+                                      const typename Store<S>::real (&f)[Q], std::integer_sequence<int, I...>)
+{
+    (((ez_group(I) == 0 ? gm : (ez_group(I) == 1 ? g0 : gp))[ez_index(I) * RC + q] = to_word<S>(f[I])), ...);
+}
+
+// pull of ring cell r: direction ii comes from r - (ex, ey) in its group's slot
+template <class S, int RC, int RX, int... I>
+__device__ inline void lds_pull_cell(const typename LdsWord<S>::type *gm, const typename LdsWord<S>::type *g0,
+                                     const typename LdsWord<S>::type *gp, int r, typename Store<S>::real (&f)[Q],
+                                     std::integer_sequence<int, I...>)
+{
+    ((f[I] = from_word<S>((ez_group(I) == 0 ? gm : (ez_group(I) == 1 ? g0 : gp))[ez_index(I) * RC + r -
+                                                                                EY[I] * RX - EX[I]])),
+     ...);
+}
+
+// PF: prefetch.  The populations of plane p + 2 are loaded into registers
+// before the barrier and the second step of plane p, and collided in the next
+// iteration, so the HBM latency of the first step hides behind the second
+// (barriers wait for LDS traffic only).  Costs 27 registers per ring cell a
+// thread owns; used where the workgroup's LDS share leaves few waves per CU.
+template <class S, int TX, int TY, int NTH, bool NT, bool PF>
+__global__ void __launch_bounds__(NTH)
+k_step2(const S *__restrict__ src, S *__restrict__ dst, const BoxDesc *__restrict__ boxes, int xcds, int zseg)
+{
+    using R = typename Store<S>::real;
+    using Wd = typename LdsWord<S>::type;
+    constexpr int RX = TX + 2, RC = RX * (TY + 2), IC = TX * TY;
+    constexpr int NC = (RC + NTH - 1) / NTH;   // ring cells per thread
+    __shared__ Wd lds[54 * RC];
+    constexpr auto seq = std::make_integer_sequence<int, Q>{};
+    const BoxDesc &d = boxes[0];
+    const int tiles_x = d.nx / TX, ntile = tiles_x * (d.ny / TY);
+    const int w = xcd_slot((int)blockIdx.x, (int)gridDim.x, xcds, 0);
+    const int tile = w % ntile, seg = w / ntile;
+    const int x0 = (tile % tiles_x) * TX, y0 = (tile / tiles_x) * TY;
+    const int z0 = seg * zseg, z1 = min(d.nz, z0 + zseg);
+    const S *sbox = src + d.base + d.zorg;
+    S *dbox = dst + d.base + d.zorg;
+    const int dsb = (int)(d.dstride * (long long)sizeof(S));
+
+    // ring cell q of plane p: loads its pulled populations into f
+    auto load_ring = [&](int p, int q, R (&f)[Q]) {
+        const int z = p < 0 ? p + d.nz : (p >= d.nz ? p - d.nz : p);
+        const int zm = z == 0 ? d.nz - 1 : z - 1, zp = z == d.nz - 1 ? 0 : z + 1;
+        const Planes<S, true> P(sbox + (long long)zm * d.sz, sbox + (long long)z * d.sz,
+                                sbox + (long long)zp * d.sz, dbox);
+        int x = x0 - 1 + q % RX, y = y0 - 1 + q / RX;
+        if (x < 0) x += d.nx; else if (x >= d.nx) x -= d.nx;
+        if (y < 0) y += d.ny; else if (y >= d.ny) y -= d.ny;
+        const Nbr n = neighbours<true>(d, x, y, z);
+        load_cell_io<S, true>(P, dsb, d.org_xy + x + y * d.sy, n, f, seq);
+    };
+    // collide ring cell q of the intermediate plane p and keep it in LDS
+    auto put_ring = [&](int p, int q, R (&f)[Q]) {
+        Wd *gm = lds, *g0 = lds + (9 + 9 * ((p + 4) & 1)) * RC, *gp = lds + (27 + 9 * ((p + 3) % 3)) * RC;
+        if constexpr (Store<S>::shifted) collide32(f, d.omega_f);
+        else collide64(f, d.omega);
+        lds_store_cell<S, RC>(gm, g0, gp, q, f, seq);
+    };
+    // A(p) for p in [z0 - 1, z1]: plane p mod nz of the intermediate step
+    auto first = [&](int p) {
+        for (int q = (int)threadIdx.x; q < RC; q += NTH) {
+            R f[Q];
+            load_ring(p, q, f);
+            put_ring(p, q, f);
+        }
+    };
+    // B(k) from A(k-1), A(k), A(k+1)
+    auto second = [&](int k) {
+        const Wd *gm = lds, *g0 = lds + (9 + 9 * ((k + 4) & 1)) * RC, *gp = lds + (27 + 9 * ((k + 2) % 3)) * RC;
+        S *wp = dbox + (long long)k * d.sz;
+        const Planes<S, true> P(wp, wp, wp, wp);
+        for (int q = (int)threadIdx.x; q < IC; q += NTH) {
+            const int tx = q % TX, ty = q / TX;
+            R f[Q];
+            lds_pull_cell<S, RC, RX>(gm, g0, gp, (ty + 1) * RX + tx + 1, f, seq);
+            if constexpr (Store<S>::shifted) collide32(f, d.omega_f);
+            else collide64(f, d.omega);
+            store_cell_io<S, NT>(P, dsb, d.org_xy + (x0 + tx) + (y0 + ty) * d.sy, f, seq);
+        }
+    };
+    first(z0 - 1);
+    first(z0);
+    if constexpr (PF) {
+        R pf[NC][Q];
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+            if ((int)threadIdx.x + c * NTH < RC) load_ring(z0 + 1, (int)threadIdx.x + c * NTH, pf[c]);
+#pragma clang loop unroll(disable)
+        for (int k = z0; k < z1; ++k) {
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+                if ((int)threadIdx.x + c * NTH < RC) put_ring(k + 1, (int)threadIdx.x + c * NTH, pf[c]);
+            if (k + 1 < z1) {
+#pragma unroll
+                for (int c = 0; c < NC; ++c)
+                    if ((int)threadIdx.x + c * NTH < RC) load_ring(k + 2, (int)threadIdx.x + c * NTH, pf[c]);
+            }
+            __syncthreads();
+            second(k);
+            __syncthreads();
+        }
+    } else {
+#pragma clang loop unroll(disable)
+        for (int k = z0; k < z1; ++k) {
+            first(k + 1);
+            __syncthreads();
+            second(k);
+            __syncthreads();
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Initial state (src/lbm.cpp:78-108) from per-axis tables [sin | cos | cos2x]
 // evaluated on the host, in FP64, stored as f (FP64) or g = f - w*rho0.
 // ---------------------------------------------------------------------------

@@ -52,6 +52,7 @@ LBM::LBM(lbm_comm comm_, int sx, int sy, int sz, double &tau, double &rho0_, dou
     cfg.ntiles = env_int("IMEXLBM_TILES", 8);
     cfg.halo_depth = env_int("IMEXLBM_HALO", 1);
     cfg.profile = env_int("IMEXLBM_PROFILE", 0);
+    cfg.two_step = env_int("IMEXLBM_TWO_STEP", -1);
 
     // one rank owns the whole domain (src/mpi_view_transfer.cpp:163-195 at rx=ry=rz=1)
     l_l[0] = glx; l_l[1] = gly; l_l[2] = glz;

@@ -82,15 +82,32 @@ typedef struct imex_config {
                              8- and 4-byte formats, 1 for 2-byte ones     */
     int contiguous;       /* 1: ask for physically contiguous population
                              pools (falls back to a plain allocation)     */
-    int io16;             /* 2-byte formats: 0 = buffer-descriptor loads
-                             and stores, 1 = global loads and stores;
-                             -1 = auto (1)                                */
+    int io16;             /* 0 = buffer-descriptor loads and stores, 1 =
+                             global loads and stores (any format);
+                             -1 = auto (1 for 2-byte formats, else 0)     */
     int profile;          /* 1: time every phase with HIP events (one
                              synchronisation per phase, graphs off)       */
     int xcd_run;          /* with xcd_groups > 1: each XCD takes runs of
                              this many consecutive work tiles; 0 = one
                              contiguous range per XCD; -1 = auto (64 for
                              4-byte formats, else 0)                      */
+    int two_step;         /* temporal blocking: 1 = advance pairs of steps
+                             in one pass (each workgroup sweeps z over a
+                             column of cells and keeps the intermediate
+                             step in LDS), 0 = one pass per step, -1 =
+                             auto (on where the domain allows it: one
+                             periodic domain, IMEX_WRAP, nx and ny
+                             multiples of the column tile)               */
+    int two_step_tile;    /* column tile: 0 = 64x8 cells (512 threads),
+                             1 = 32x16 (512), 2 = 32x8 (256), 3 = 32x8
+                             (384); -1 = auto (0 for 2-byte, 1 for 4-byte,
+                             2 for FP64; FP64 builds only 2 and 3)        */
+    int two_step_zseg;    /* z-planes per workgroup sweep; 0 = auto       */
+    int two_step_xcds;    /* tile order of the two-step pass over the
+                             XCDs, as xcd_groups; 0 = auto               */
+    int two_step_prefetch; /* 8- and 4-byte formats: load the next plane's
+                             populations into registers ahead of the
+                             barrier; 1 = on, 0 = off, -1 = auto (on)    */
 } imex_config;
 
 typedef struct imex_info {
@@ -109,6 +126,8 @@ typedef struct imex_info {
     int contiguous;           /* 1 if the pools were allocated contiguous */
     int io16;
     int xcd_run;
+    int two_step;             /* 1 if pairs of steps run as one pass      */
+    int two_step_tile, two_step_zseg, two_step_xcds, two_step_prefetch;
 } imex_info;
 
 /* Defaults for every field of imex_config: FP64, wrap, 256-thread

@@ -140,3 +140,15 @@ def test_driver_phase_profile(tmp_path):
     assert float(rows["fused pull-stream + collide"][5]) > 0      # GB/s
     for a, b in zip(ref, got):
         assert abs(b[1] - a[1]) / a[1] <= 1e-12 and abs(b[5] - a[5]) / a[5] <= 1e-12
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp16"])
+def test_driver_two_step_pairs_match_single_steps(tmp_path, prec):
+    # reports every 5 steps: the driver's Collision/Streaming calls go out as
+    # two-step passes with a single step left over per interval; the cons
+    # lines must equal the one-pass-per-step run bit for bit
+    deck = "1.0  0.1  1600\n0.05  40  5\n64  64  32\n"
+    on, _ = _run(tmp_path, deck, IMEXLBM_PRECISION=prec, IMEXLBM_TWO_STEP=1)
+    off, _ = _run(tmp_path, deck, IMEXLBM_PRECISION=prec, IMEXLBM_TWO_STEP=0)
+    assert len(on) == len(off) == 8
+    assert on == off

@@ -1,0 +1,172 @@
+"""Two steps per pass (temporal blocking) against one pass per step.
+
+The two-step kernel keeps the intermediate step in LDS rounded to the storage
+format, so its populations must be bit-identical to two single-step launches
+in every format, for every column tile, z-segmentation and tile order, and
+through every entry point that advances the state (imex_step, the deferred
+Collision/Streaming pairs, graph replay, the autotuner).  FP64 is also checked
+directly against the oracle.
+"""
+import numpy as np
+import pytest
+
+from tests.conftest import ROOT  # noqa: F401
+
+pytestmark = pytest.mark.gpu
+
+
+def _dom(nx, ny=None, nz=None, re=400, u0=0.05, rho0=1.0):
+    from oracle import oracle
+    return {"nx": nx, "ny": ny or nx, "nz": nz or nx, "tau0": oracle.tau(u0, nx, re), "rho0": rho0, "u0": u0}
+
+
+def _ctx(d, **kw):
+    from imexlbm_amd import capi
+    print(f"case: {d} {kw}", flush=True)
+    return capi.Context(d, **kw)
+
+
+def _pair(d, prec, steps, **two):
+    """get_f and conserved after `steps` steps, two-step on (with `two`) and off."""
+    with _ctx(d, precision=prec, two_step=1, **two) as a:
+        a.initialize()
+        a.step(steps)
+        info = a.info()
+        fa, ca = a.get_f(), a.conserved()
+    with _ctx(d, precision=prec, two_step=0) as b:
+        b.initialize()
+        b.step(steps)
+        assert b.info()["two_step"] == 0
+        fb, cb = b.get_f(), b.conserved()
+    assert info["two_step"] == 1
+    return fa, fb, ca, cb
+
+
+CASES = ([("fp64", t, pf) for t in (2, 3) for pf in (0, 1)] +
+         [("fp32", t, pf) for t in (0, 1, 2, 3) for pf in (0, 1)] +
+         [(p, t, 0) for p in ("fp16", "bf16") for t in (0, 1, 2, 3)])
+
+
+@pytest.mark.parametrize("prec,tile,pf", CASES)
+@pytest.mark.parametrize("shape,zseg,steps", [((64, 64, 64), 0, 7), ((128, 64, 24), 7, 6),
+                                              ((64, 32, 3), 2, 5), ((64, 16, 1), 0, 4)])
+def test_two_step_bitwise_equals_single_steps(prec, tile, pf, shape, zseg, steps):
+    tx, ty = {0: (64, 8), 1: (32, 16), 2: (32, 8), 3: (32, 8)}[tile]
+    if shape[0] % tx or shape[1] % ty:
+        pytest.skip("tile does not divide the domain")
+    d = _dom(*shape)
+    fa, fb, ca, cb = _pair(d, prec, steps, two_step_tile=tile, two_step_zseg=zseg, two_step_prefetch=pf)
+    assert np.array_equal(fa, fb)
+    assert np.array_equal(np.array(ca), np.array(cb))
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp16"])
+def test_two_step_tile_order(prec):
+    d = _dom(64, nz=20)
+    tile = 2 if prec == "fp64" else 0
+    outs = []
+    for x in (1, 8):
+        fa, fb, _, _ = _pair(d, prec, 9, two_step_tile=tile, two_step_zseg=6, two_step_xcds=x)
+        assert np.array_equal(fa, fb)
+        outs.append(fa)
+    assert np.array_equal(outs[0], outs[1])
+
+
+def test_two_step_fp64_matches_oracle_bitwise():
+    from oracle import oracle
+    d = _dom(32, re=1600)
+    with _ctx(d, precision="fp64") as c:      # auto: on for 32^3 FP64
+        c.initialize()
+        assert c.info()["two_step"] == 1
+        c.step(40)
+        got = c.get_f()
+    o = oracle.Oracle(d["nx"], d["ny"], d["nz"], d["tau0"], d["rho0"], d["u0"])
+    o.initialize()
+    o.step(40)
+    want = o.get_f()
+    o.close()
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16"])
+def test_deferred_collision_streaming_pairs(prec):
+    # the driver's call pattern: Collision + Streaming per step, reads at odd
+    # and even points; Streaming defers one fused step and launches pairs
+    d = _dom(64, nz=16)
+    with _ctx(d, precision=prec, two_step=1) as a, _ctx(d, precision=prec, two_step=0) as b:
+        a.initialize()
+        b.initialize()
+        for it in range(1, 12):
+            a.collision(); a.streaming()
+            b.collision(); b.streaming()
+            if it in (3, 4, 7, 11):
+                assert np.array_equal(np.array(a.conserved()), np.array(b.conserved()))
+            if it == 8:
+                a.synchronize()
+        # a pending collision, a bare streaming and the get on top
+        a.collision(); b.collision()
+        assert np.array_equal(a.get_f(), b.get_f())
+        a.streaming(); b.streaming()
+        a.streaming(); b.streaming()
+        assert np.array_equal(a.get_f(), b.get_f())
+
+
+def test_two_step_graph_replay():
+    d = _dom(64, nz=12)
+    with _ctx(d, precision="fp32", two_step=1, use_graph=1) as a, _ctx(d, precision="fp32", two_step=0) as b:
+        a.initialize(); b.initialize()
+        a.step(1); b.step(1)
+        for k in range(4):
+            a.step(6); b.step(6)
+            a.synchronize()
+            assert np.array_equal(a.get_f(), b.get_f())
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp16"])
+def test_autotune_with_two_step_candidates(prec):
+    d = _dom(64, nz=32)
+    with _ctx(d, precision=prec) as a, _ctx(d, precision=prec, two_step=0) as b:
+        a.initialize(); b.initialize()
+        a.step(3); b.step(3)
+        before = a.get_f()
+        assert a.autotune(reps=2) > 0
+        assert np.array_equal(a.get_f(), before)
+        a.step(9); b.step(9)
+        assert np.array_equal(a.get_f(), b.get_f())
+
+
+def test_two_step_eligibility():
+    from imexlbm_amd import capi
+    d = _dom(48, ny=40, nz=8)
+    with _ctx(d, precision="fp32") as c:          # 48 is no multiple of 32: auto stays off
+        assert c.info()["two_step"] == 0
+    with pytest.raises(capi.ImexError):
+        _ctx(d, precision="fp32", two_step=1)
+    with pytest.raises(capi.ImexError):
+        _ctx(_dom(64), precision="fp64", two_step=1, two_step_tile=0)   # FP64 builds only 32x8
+    with pytest.raises(capi.ImexError):
+        _ctx(_dom(64), precision="fp32", two_step=1, mode=capi.TILED, ntiles=2)
+    with _ctx(_dom(64), precision="fp32", mode=capi.TILED, ntiles=2) as c:
+        assert c.info()["two_step"] == 0
+
+
+def test_two_step_full_size_fp16_512():
+    # the bench configuration: conserved sums of the two paths bit-identical
+    # after 1 + 4 steps (each context holds 29 GB)
+    from imexlbm_amd import capi
+    n = 512
+    d = {"nx": n, "ny": n, "nz": n, "tau0": 0.05 * n / 1600 / (1.0 / 3.0) / 2.0 / 3.1415926,
+         "rho0": 1.0, "u0": 0.05}
+    out = []
+    for two in (1, 0):
+        with _ctx(d, precision="fp16", two_step=two) as c:
+            c.initialize()
+            c.step(5)
+            out.append(np.array(c.conserved()))
+            m = c.get_macro()
+            out.append(np.array([float(np.sum(a, dtype=np.float64)) for a in m]))
+            assert np.all(np.isfinite(out[-1]))
+            assert c.info()["two_step"] == two
+            del m
+    assert np.array_equal(out[0], out[2])
+    assert np.array_equal(out[1], out[3])
