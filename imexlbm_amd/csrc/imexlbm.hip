@@ -467,19 +467,21 @@ int launch_step(imex_ctx *c, int op, int src, int dst)
 }
 
 // Two-step pass: column tile {TX, TY} of each tile variant (3 is the 32x8
-// tile with 384 threads: one ring cell per thread).
+// tile with 384 threads: one ring cell per thread; 4 the 64x16 tile with
+// 1024 threads, 2-byte formats only: its ring takes 128 KB of LDS).
 void ts_dims(int tile, int &tx, int &ty)
 {
-    tx = tile == 0 ? 64 : 32;
-    ty = tile == 1 ? 16 : 8;
+    tx = (tile == 0 || tile == 4) ? 64 : 32;
+    ty = (tile == 1 || tile == 4) ? 16 : 8;
 }
 
 // One periodic domain in wrap mode whose x and y extents the column tile
 // divides; FP64 builds only the 32x8 tiles (its LDS ring needs 147 KB).
 bool ts_allowed(const imex_ctx *c, int tile)
 {
-    if (c->tiled || c->inst.size() != 1 || c->boxes.size() != 1 || tile < 0 || tile > 3) return false;
-    if (c->ssize == 8 && tile < 2) return false;
+    if (c->tiled || c->inst.size() != 1 || c->boxes.size() != 1 || tile < 0 || tile > 4) return false;
+    if (c->ssize == 8 && (tile < 2 || tile == 4)) return false;
+    if (c->ssize == 4 && tile == 4) return false;
     int tx, ty;
     ts_dims(tile, tx, ty);
     return c->boxes[0].nx % tx == 0 && c->boxes[0].ny % ty == 0;
@@ -510,13 +512,11 @@ int launch_ts(imex_ctx *c, int src, int dst)
     if (grid > 0x7fffffffLL) return fail(IMEX_ERANGE, "two-step grid too large");
     const S *sp = (const S *)pool_ptr(c, src);
     S *dp = (S *)pool_ptr(c, dst);
-    if constexpr (sizeof(S) >= 4) {
-        if (c->ts_pf) {
-            hipLaunchKernelGGL((k_step2<S, TX, TY, NTH, true, true>), dim3((unsigned)grid), dim3(NTH), 0, c->stream,
-                               sp, dp, c->d_boxes, c->ts_xcds, zseg);
-            HIPC(hipGetLastError());
-            return IMEX_OK;
-        }
+    if (c->ts_pf) {
+        hipLaunchKernelGGL((k_step2<S, TX, TY, NTH, true, true>), dim3((unsigned)grid), dim3(NTH), 0, c->stream,
+                           sp, dp, c->d_boxes, c->ts_xcds, zseg);
+        HIPC(hipGetLastError());
+        return IMEX_OK;
     }
     hipLaunchKernelGGL((k_step2<S, TX, TY, NTH, true, false>), dim3((unsigned)grid), dim3(NTH), 0, c->stream,
                        sp, dp, c->d_boxes, c->ts_xcds, zseg);
@@ -535,6 +535,9 @@ int launch_step2_t(imex_ctx *c, int src, int dst)
         case 0: return launch_ts<S, 64, 8, 512>(c, src, dst);
         case 1: return launch_ts<S, 32, 16, 512>(c, src, dst);
         case 3: return launch_ts<S, 32, 8, 384>(c, src, dst);
+        case 4:
+            if constexpr (sizeof(S) == 2) return launch_ts<S, 64, 16, 1024>(c, src, dst);
+            return fail(IMEX_EINVAL, "two-step tile 4 is built for 2-byte formats only");
         default: return launch_ts<S, 32, 8, 256>(c, src, dst);
         }
     }
@@ -971,7 +974,7 @@ int imex_create(imex_ctx **out, const imex_config *cfg_in, const imex_domain *do
         c->two_step = (cfg.two_step != 0 && ok) ? 1 : 0;
         c->ts_zseg = cfg.two_step_zseg > 0 ? cfg.two_step_zseg : (ok ? ts_default_zseg(c, c->ts_tile) : 1);
         c->ts_xcds = cfg.two_step_xcds > 0 ? cfg.two_step_xcds : (c->ssize == 2 ? 1 : 8);
-        c->ts_pf = c->ssize >= 4 && cfg.two_step_prefetch != 0 ? 1 : 0;
+        c->ts_pf = cfg.two_step_prefetch == 1 || (cfg.two_step_prefetch < 0 && c->ssize >= 4) ? 1 : 0;
     }
     if (c->tiled) build_halo(c);
 
@@ -1306,9 +1309,9 @@ int imex_autotune(imex_ctx *c, int reps, double *best_ms)
         struct Cand2 { int tile, zseg, xcds, pf; };
         std::vector<Cand2> c2;
         const int nz = c->boxes.empty() ? 0 : c->boxes[0].nz;
-        const int pf_max = (c->ssize >= 4 && c->cfg.two_step_prefetch != 0) ? 1 : 0;
-        const int pf_min = (c->cfg.two_step_prefetch == 1 && pf_max) ? 1 : 0;
-        for (int tile : {0, 1, 2, 3}) {
+        const int pf_max = c->cfg.two_step_prefetch != 0 ? 1 : 0;
+        const int pf_min = c->cfg.two_step_prefetch == 1 ? 1 : 0;
+        for (int tile : {0, 1, 2, 3, 4}) {
             if (!ts_allowed(c, tile)) continue;
             std::vector<int> zs = {ts_default_zseg(c, tile)};
             for (int div : {1, 2, 4, 8, 16})

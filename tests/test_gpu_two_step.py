@@ -44,14 +44,14 @@ def _pair(d, prec, steps, **two):
 
 CASES = ([("fp64", t, pf) for t in (2, 3) for pf in (0, 1)] +
          [("fp32", t, pf) for t in (0, 1, 2, 3) for pf in (0, 1)] +
-         [(p, t, 0) for p in ("fp16", "bf16") for t in (0, 1, 2, 3)])
+         [(p, t, pf) for p in ("fp16", "bf16") for t in (0, 1, 2, 3, 4) for pf in (0, 1)])
 
 
 @pytest.mark.parametrize("prec,tile,pf", CASES)
 @pytest.mark.parametrize("shape,zseg,steps", [((64, 64, 64), 0, 7), ((128, 64, 24), 7, 6),
                                               ((64, 32, 3), 2, 5), ((64, 16, 1), 0, 4)])
 def test_two_step_bitwise_equals_single_steps(prec, tile, pf, shape, zseg, steps):
-    tx, ty = {0: (64, 8), 1: (32, 16), 2: (32, 8), 3: (32, 8)}[tile]
+    tx, ty = {0: (64, 8), 1: (32, 16), 2: (32, 8), 3: (32, 8), 4: (64, 16)}[tile]
     if shape[0] % tx or shape[1] % ty:
         pytest.skip("tile does not divide the domain")
     d = _dom(*shape)
