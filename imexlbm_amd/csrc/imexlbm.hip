@@ -487,20 +487,16 @@ bool ts_allowed(const imex_ctx *c, int tile)
     return c->boxes[0].nx % tx == 0 && c->boxes[0].ny % ty == 0;
 }
 
-// Enough z-segments that the pass has about 8 workgroups per CU (2-byte
-// formats, two 71 KB workgroups fit a CU) or one per CU (others), and at
-// least 32 planes per sweep where the box has them: each sweep recomputes
-// two intermediate planes.
-int ts_default_zseg(const imex_ctx *c, int tile)
+// Planes per z-sweep: more sweeps give more workgroups, and with the tile
+// order by XCD each XCD a z-range of its own, but every sweep recomputes two
+// intermediate planes.  Autotuned optima at 256^3 and 512^3 lie near nz/8
+// (at least 32) for 2-byte formats and near nz/4 (at least 64) for the
+// others, whose passes carry more work per plane.
+int ts_default_zseg(const imex_ctx *c, int /*tile*/)
 {
-    int tx, ty;
-    ts_dims(tile, tx, ty);
-    const BoxDesc &b = c->boxes[0];
-    const long long ntile = (long long)(b.nx / tx) * (b.ny / ty);
-    const long long want = (c->ssize == 2 ? 8LL : 1LL) * c->ncu;
-    const long long nseg = std::max(1LL, (want + ntile - 1) / ntile);
-    const int zseg = (int)((b.nz + nseg - 1) / nseg);
-    return std::max(zseg, std::min(32, b.nz));
+    const int nz = c->boxes[0].nz;
+    const int z = c->ssize == 2 ? std::max(32, nz / 8) : std::max(64, nz / 4);
+    return std::min(z, nz);
 }
 
 template <class S, int TX, int TY, int NTH>
@@ -962,10 +958,10 @@ int imex_create(imex_ctx **out, const imex_config *cfg_in, const imex_domain *do
     if (r != IMEX_OK) return bail(r);
     build_tiles(c);
     {
-        const int def_tile = c->ssize == 2 ? 0 : (c->ssize == 4 ? 1 : 2);
+        const int def_tile = c->ssize == 8 ? 3 : 0;
         c->ts_tile = cfg.two_step_tile >= 0 ? cfg.two_step_tile : def_tile;
         if (!ts_allowed(c, c->ts_tile) && cfg.two_step_tile < 0)
-            for (int t : {0, 1, 2})
+            for (int t : {0, 1, 3, 2, 4})
                 if (ts_allowed(c, t)) { c->ts_tile = t; break; }
         const bool ok = ts_allowed(c, c->ts_tile);
         if (cfg.two_step == 1 && !ok)
@@ -973,7 +969,7 @@ int imex_create(imex_ctx **out, const imex_config *cfg_in, const imex_domain *do
                                           "column tile divides (FP64: 32x8 only)"));
         c->two_step = (cfg.two_step != 0 && ok) ? 1 : 0;
         c->ts_zseg = cfg.two_step_zseg > 0 ? cfg.two_step_zseg : (ok ? ts_default_zseg(c, c->ts_tile) : 1);
-        c->ts_xcds = cfg.two_step_xcds > 0 ? cfg.two_step_xcds : (c->ssize == 2 ? 1 : 8);
+        c->ts_xcds = cfg.two_step_xcds > 0 ? cfg.two_step_xcds : 8;
         c->ts_pf = cfg.two_step_prefetch == 1 || (cfg.two_step_prefetch < 0 && c->ssize >= 4) ? 1 : 0;
     }
     if (c->tiled) build_halo(c);

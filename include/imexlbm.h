@@ -101,11 +101,12 @@ typedef struct imex_config {
     int two_step_tile;    /* column tile: 0 = 64x8 cells (512 threads),
                              1 = 32x16 (512), 2 = 32x8 (256), 3 = 32x8
                              (384), 4 = 64x16 (1024, 2-byte formats only);
-                             -1 = auto (0 for 2-byte, 1 for 4-byte, 2 for
-                             FP64; FP64 builds only 2 and 3)              */
+                             -1 = auto (3 for FP64, else 0; FP64 builds
+                             only 2 and 3)                                */
     int two_step_zseg;    /* z-planes per workgroup sweep; 0 = auto       */
     int two_step_xcds;    /* tile order of the two-step pass over the
-                             XCDs, as xcd_groups; 0 = auto               */
+                             XCDs, as xcd_groups; 0 = auto (8: each XCD
+                             sweeps its own z-range)                     */
     int two_step_prefetch; /* load the next plane's populations into
                              registers ahead of the barrier; 1 = on, 0 =
                              off, -1 = auto (on for 8- and 4-byte formats) */
