@@ -46,11 +46,12 @@ def tau0(u0, n, re):
     return u0 * n / re / (1.0 / 3.0) / 2.0 / PI_LIT
 
 
-def run_mode(capi, prec, n, re, u0, steps, warmup, autotune, use_graph, mode="wrap", tiles=8, halo=1):
+def run_mode(capi, prec, n, re, u0, steps, warmup, autotune, use_graph, mode="wrap", tiles=8, halo=1,
+             two_step=-1):
     dom = {"nx": n, "ny": n, "nz": n, "tau0": tau0(u0, n, re), "rho0": 1.0, "u0": u0}
-    kw = {}
+    kw = {"two_step": two_step}
     if mode == "tiled":
-        kw = {"mode": capi.TILED, "ntiles": tiles, "halo_depth": halo}
+        kw.update({"mode": capi.TILED, "ntiles": tiles, "halo_depth": halo})
     with capi.Context(dom, precision=prec, use_graph=int(use_graph), **kw) as ctx:
         ctx.initialize()
         ctx.step(1)                       # first collision: the state becomes post-collision
@@ -168,6 +169,8 @@ def main():
                     help="periodic wrap in the kernel, or subdomain tiles with device pack/unpack")
     ap.add_argument("--tiles", type=int, default=8)
     ap.add_argument("--halo", type=int, default=1, help="ghost layers filled per exchange (tiled)")
+    ap.add_argument("--two-step", choices=("auto", "on", "off"), default="auto",
+                    help="two steps per pass (temporal blocking): auto lets the autotuner decide")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -187,7 +190,8 @@ def main():
     for m in modes:
         print(f"bench: {m} {args.n}^3 steps={args.steps} warmup={args.warmup}", file=sys.stderr, flush=True)
         results[m] = run_mode(capi, m, args.n, args.re, args.u0, args.steps, args.warmup,
-                              not args.no_autotune, args.graph, args.mode, args.tiles, args.halo)
+                              not args.no_autotune, args.graph, args.mode, args.tiles, args.halo,
+                              {"auto": -1, "on": 1, "off": 0}[args.two_step])
         print(f"bench: {m} {results[m]['mlups']} MLUPS, {results[m]['achieved_gbs']} GB/s",
               file=sys.stderr, flush=True)
 

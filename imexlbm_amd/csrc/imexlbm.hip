@@ -124,14 +124,14 @@ struct imex_ctx {
     std::vector<int> box_coord;  // tile coordinates, 3 per box (tiled mode)
     std::vector<WorkTile> tiles;
     std::vector<HaloSeg> segs;
-    std::vector<int2> pack_blocks, unpack_blocks;
+    std::vector<int2> pack_blocks, unpack_blocks, direct_blocks;
     int rgrid[3] = {1, 1, 1};
 
     BoxDesc *d_boxes = nullptr;
     WorkTile *d_tiles = nullptr;
     size_t d_tiles_cap = 0;
     HaloSeg *d_segs = nullptr;
-    int2 *d_pack = nullptr, *d_unpack = nullptr;
+    int2 *d_pack = nullptr, *d_unpack = nullptr, *d_direct = nullptr;
     double *d_tabs = nullptr;
     double *d_partial = nullptr;
     double *d_out = nullptr;
@@ -155,6 +155,12 @@ struct imex_ctx {
     int xcd_run = 0;
     // two steps per pass: on/off, column tile, z-planes per sweep, tile order
     int two_step = 0, ts_tile = 0, ts_zseg = 1, ts_xcds = 1, ts_pf = 0;
+    bool halo_all = false;       // tiled two-step: ghosts carry all 27 directions
+    bool halo_direct = true;     // tiled: one direct copy instead of pack + unpack
+    long long direct_bytes = 0;  // bytes the direct fill moves
+    PassBox *d_pass = nullptr;   // per-box workgroup table of the two-step pass
+    int pass_key[2] = {-1, -1};  // tile and z-sweep the table was built for
+    long long pass_grid = 0;
     int deferred = 0;            // fused steps owed by Streaming() calls, < 2
     hipGraph_t graph[2] = {nullptr, nullptr};
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
@@ -340,6 +346,8 @@ void build_halo(imex_ctx *c)
     c->segs.clear();
     c->pack_blocks.clear();
     c->unpack_blocks.clear();
+    c->direct_blocks.clear();
+    std::vector<HaloSeg> direct;
     const int d = c->halo_depth;
     long long xoff = 0;
     std::vector<HaloSeg> unpack;
@@ -366,7 +374,7 @@ void build_halo(imex_ctx *c)
                             else if (ch[a] > 0) { s0[a] = n[a] - d; d0[a] = -d; ext[a] = d; }
                             else { s0[a] = 0; d0[a] = nt[a]; ext[a] = d; }
                         }
-                        const unsigned mask = channel_mask(ch);
+                        const unsigned mask = c->halo_all ? (1u << Q) - 1 : channel_mask(ch);
                         const int ndir = __builtin_popcount(mask);
                         HaloSeg ps{};
                         ps.ex = ext[0]; ps.ey = ext[1]; ps.ez = ext[2];
@@ -388,8 +396,14 @@ void build_halo(imex_ctx *c)
                         us.d_ds = t.dstride;
                         us.d_sy = t.sy; us.d_sz = t.sz;
                         xoff += round_up((long long)ndir * ps.ncell, 64);
+                        HaloSeg ds = ps;   // boundary slab of b -> ghost slab of t
+                        ds.pack = 2;
+                        ds.d_off = us.d_off;
+                        ds.d_ds = us.d_ds;
+                        ds.d_sy = us.d_sy; ds.d_sz = us.d_sz;
                         c->segs.push_back(ps);
                         unpack.push_back(us);
+                        direct.push_back(ds);
                     }
         }
     }
@@ -399,10 +413,18 @@ void build_halo(imex_ctx *c)
     for (int s = 0; s < (int)unpack.size(); ++s)
         for (int c0 = 0; c0 < unpack[s].ncell; c0 += 256) c->unpack_blocks.push_back({npack + s, c0});
     c->segs.insert(c->segs.end(), unpack.begin(), unpack.end());
+    const int nsu = (int)c->segs.size();
+    for (int s = 0; s < (int)direct.size(); ++s)
+        for (int c0 = 0; c0 < direct[s].ncell; c0 += 256) c->direct_blocks.push_back({nsu + s, c0});
+    c->segs.insert(c->segs.end(), direct.begin(), direct.end());
     c->xbuf_elems = xoff;
     c->halo_bytes = 0;
-    for (const HaloSeg &sg : c->segs)   // each segment reads and writes its cells
-        c->halo_bytes += 2LL * __builtin_popcount(sg.mask) * sg.ncell * c->ssize;
+    c->direct_bytes = 0;
+    for (const HaloSeg &sg : c->segs) {   // each segment reads and writes its cells
+        const long long b = 2LL * __builtin_popcount(sg.mask) * sg.ncell * c->ssize;
+        if (sg.pack == 2) c->direct_bytes += b;
+        else c->halo_bytes += b;
+    }
 }
 
 // --------------------------------------------------------------------------
@@ -475,16 +497,45 @@ void ts_dims(int tile, int &tx, int &ty)
     ty = (tile == 1 || tile == 4) ? 16 : 8;
 }
 
-// One periodic domain in wrap mode whose x and y extents the column tile
-// divides; FP64 builds only the 32x8 tiles (its LDS ring needs 147 KB).
+// One domain (one periodic box, or the tiles of a tiled domain, at least two
+// planes deep) whose boxes' x and y extents the column tile divides; FP64
+// builds only the 32x8 tiles (its LDS ring needs 147 KB).
 bool ts_allowed(const imex_ctx *c, int tile)
 {
-    if (c->tiled || c->inst.size() != 1 || c->boxes.size() != 1 || tile < 0 || tile > 4) return false;
+    if (c->inst.size() != 1 || tile < 0 || tile > 4) return false;
     if (c->ssize == 8 && (tile < 2 || tile == 4)) return false;
     if (c->ssize == 4 && tile == 4) return false;
     int tx, ty;
     ts_dims(tile, tx, ty);
-    return c->boxes[0].nx % tx == 0 && c->boxes[0].ny % ty == 0;
+    for (const BoxDesc &b : c->boxes)
+        if (b.nx % tx || b.ny % ty || (c->tiled && b.nz < 2)) return false;
+    return true;
+}
+
+// Per-box workgroup table of the two-step pass for the current tile and
+// z-sweep length; rebuilt (synchronously, outside any capture) when they
+// change.
+int ts_prepare(imex_ctx *c)
+{
+    if (c->pass_key[0] == c->ts_tile && c->pass_key[1] == c->ts_zseg) return IMEX_OK;
+    if (c->capturing) return fail(IMEX_ESTATE, "two-step table changed under capture");
+    int tx, ty;
+    ts_dims(c->ts_tile, tx, ty);
+    std::vector<PassBox> t;
+    long long wg = 0;
+    for (int bi = 0; bi < (int)c->boxes.size(); ++bi) {
+        const BoxDesc &b = c->boxes[bi];
+        const int zseg = std::max(1, std::min(c->ts_zseg, b.nz));
+        const int tiles_x = b.nx / tx, ntile = tiles_x * (b.ny / ty);
+        t.push_back({(int)wg, bi, tiles_x, ntile});
+        wg += (long long)ntile * ((b.nz + zseg - 1) / zseg);
+    }
+    if (wg > 0x7fffffffLL) return fail(IMEX_ERANGE, "two-step grid too large");
+    HIPC(hipMemcpy(c->d_pass, t.data(), t.size() * sizeof(PassBox), hipMemcpyHostToDevice));
+    c->pass_grid = wg;
+    c->pass_key[0] = c->ts_tile;
+    c->pass_key[1] = c->ts_zseg;
+    return IMEX_OK;
 }
 
 // Planes per z-sweep: more sweeps give more workgroups, and with the tile
@@ -502,20 +553,19 @@ int ts_default_zseg(const imex_ctx *c, int /*tile*/)
 template <class S, int TX, int TY, int NTH>
 int launch_ts(imex_ctx *c, int src, int dst)
 {
-    const BoxDesc &b = c->boxes[0];
-    const int zseg = std::max(1, std::min(c->ts_zseg, b.nz));
-    const long long grid = (long long)(b.nx / TX) * (b.ny / TY) * ((b.nz + zseg - 1) / zseg);
-    if (grid > 0x7fffffffLL) return fail(IMEX_ERANGE, "two-step grid too large");
+    TRY(ts_prepare(c));
+    const dim3 g((unsigned)c->pass_grid), b(NTH);
     const S *sp = (const S *)pool_ptr(c, src);
     S *dp = (S *)pool_ptr(c, dst);
-    if (c->ts_pf) {
-        hipLaunchKernelGGL((k_step2<S, TX, TY, NTH, true, true>), dim3((unsigned)grid), dim3(NTH), 0, c->stream,
-                           sp, dp, c->d_boxes, c->ts_xcds, zseg);
-        HIPC(hipGetLastError());
-        return IMEX_OK;
+    const int npb = (int)c->boxes.size(), zseg = std::max(1, c->ts_zseg);
+#define TS(W, P) hipLaunchKernelGGL((k_step2<S, W, TX, TY, NTH, true, P>), g, b, 0, c->stream, sp, dp, c->d_boxes, \
+                                    c->d_pass, npb, c->ts_xcds, zseg)
+    if (c->tiled) {
+        if (c->ts_pf) TS(false, true); else TS(false, false);
+    } else {
+        if (c->ts_pf) TS(true, true); else TS(true, false);
     }
-    hipLaunchKernelGGL((k_step2<S, TX, TY, NTH, true, false>), dim3((unsigned)grid), dim3(NTH), 0, c->stream,
-                       sp, dp, c->d_boxes, c->ts_xcds, zseg);
+#undef TS
     HIPC(hipGetLastError());
     return IMEX_OK;
 }
@@ -557,6 +607,12 @@ int launch_halo_t(imex_ctx *c, int which)
 {
     S *f = (S *)pool_ptr(c, which);
     S *x = (S *)c->xbuf;
+    if (c->halo_direct) {
+        const int nd = (int)c->direct_blocks.size();
+        if (nd) hipLaunchKernelGGL((k_halo<S>), dim3(nd), dim3(256), 0, c->stream, f, f, c->d_segs, c->d_direct, nd);
+        HIPC(hipGetLastError());
+        return IMEX_OK;
+    }
     const int np = (int)c->pack_blocks.size(), nu = (int)c->unpack_blocks.size();
     if (np) hipLaunchKernelGGL((k_halo<S>), dim3(np), dim3(256), 0, c->stream, f, x, c->d_segs, c->d_pack, np);
     if (nu) hipLaunchKernelGGL((k_halo<S>), dim3(nu), dim3(256), 0, c->stream, x, f, c->d_segs, c->d_unpack, nu);
@@ -567,7 +623,7 @@ int launch_halo_t(imex_ctx *c, int which)
 int ensure_ghosts(imex_ctx *c)
 {
     if (!c->tiled || c->ghosts_valid) return IMEX_OK;
-    PhaseScope ps(c, PH_HALO, (double)c->halo_bytes);
+    PhaseScope ps(c, PH_HALO, (double)(c->halo_direct ? c->direct_bytes : c->halo_bytes));
     switch (c->prec) {
     case IMEX_F64: TRY(launch_halo_t<double>(c, c->cur)); break;
     case IMEX_F32: TRY(launch_halo_t<float>(c, c->cur)); break;
@@ -596,6 +652,7 @@ int do_fused(imex_ctx *c)
 
 int do_fused2(imex_ctx *c)
 {
+    TRY(ensure_ghosts(c));
     TRY(launch_step2(c, c->cur, 1 - c->cur));
     c->cur = 1 - c->cur;
     c->ghosts_valid = false;
@@ -668,6 +725,7 @@ int build_graph(imex_ctx *c, int parity)
     const bool save_gv = c->ghosts_valid;
     c->cur = parity;
     c->ghosts_valid = false;
+    if (c->two_step) TRY(ts_prepare(c));
     HIPC(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     c->capturing = true;
     const int r = do_pair(c);
@@ -841,6 +899,7 @@ void imex_config_default(imex_config *cfg)
     cfg->two_step = -1;
     cfg->two_step_tile = -1;
     cfg->two_step_prefetch = -1;
+    cfg->halo_direct = -1;
 }
 
 const char *imex_strerror(int s)
@@ -874,7 +933,7 @@ void imex_destroy(imex_ctx *c)
     drop_graphs(c);
     for (void *p : {(void *)c->d_boxes, (void *)c->d_tiles, (void *)c->d_segs, (void *)c->d_pack,
                     (void *)c->d_unpack, (void *)c->d_tabs, (void *)c->d_partial, (void *)c->d_out,
-                    (void *)c->d_stage, c->pool[0], c->pool[1], c->xbuf})
+                    (void *)c->d_stage, (void *)c->d_pass, (void *)c->d_direct, c->pool[0], c->pool[1], c->xbuf})
         if (p) (void)hipFree(p);
     if (c->h_out) (void)hipHostFree(c->h_out);
     for (hipEvent_t e : c->ev)
@@ -917,6 +976,7 @@ int imex_create(imex_ctx **out, const imex_config *cfg_in, const imex_domain *do
     c->ssize = storage_bytes(cfg.precision);
     c->tiled = cfg.mode == IMEX_TILED;
     c->halo_depth = cfg.halo_depth;
+    c->halo_direct = cfg.halo_direct != 0;
     c->block = cfg.block ? cfg.block : 256;
     c->cpt = cfg.cells_per_thread ? cfg.cells_per_thread : 2;
     c->persistent = std::max(cfg.persistent, 0);
@@ -965,9 +1025,15 @@ int imex_create(imex_ctx **out, const imex_config *cfg_in, const imex_domain *do
                 if (ts_allowed(c, t)) { c->ts_tile = t; break; }
         const bool ok = ts_allowed(c, c->ts_tile);
         if (cfg.two_step == 1 && !ok)
-            return bail(fail(IMEX_EINVAL, "two_step needs one periodic domain (IMEX_WRAP) whose nx and ny the "
-                                          "column tile divides (FP64: 32x8 only)"));
+            return bail(fail(IMEX_EINVAL, "two_step needs one domain whose boxes' nx and ny the column tile "
+                                          "divides (tiled: boxes at least 2 planes deep; FP64: 32x8 tiles only)"));
         c->two_step = (cfg.two_step != 0 && ok) ? 1 : 0;
+        if (c->two_step && c->tiled) {
+            // the second step pulls through the first step's ring: two ghost
+            // layers, with every direction of the first one
+            c->halo_all = true;
+            c->halo_depth = std::max(c->halo_depth, 2);
+        }
         c->ts_zseg = cfg.two_step_zseg > 0 ? cfg.two_step_zseg : (ok ? ts_default_zseg(c, c->ts_tile) : 1);
         c->ts_xcds = cfg.two_step_xcds > 0 ? cfg.two_step_xcds : 8;
         c->ts_pf = cfg.two_step_prefetch == 1 || (cfg.two_step_prefetch < 0 && c->ssize >= 4) ? 1 : 0;
@@ -994,11 +1060,13 @@ int imex_create(imex_ctx **out, const imex_config *cfg_in, const imex_domain *do
         const size_t sb = c->segs.size() * sizeof(HaloSeg);
         if (hipMalloc(&c->d_segs, sb) != hipSuccess ||
             hipMalloc(&c->d_pack, std::max<size_t>(c->pack_blocks.size(), 1) * sizeof(int2)) != hipSuccess ||
-            hipMalloc(&c->d_unpack, std::max<size_t>(c->unpack_blocks.size(), 1) * sizeof(int2)) != hipSuccess)
+            hipMalloc(&c->d_unpack, std::max<size_t>(c->unpack_blocks.size(), 1) * sizeof(int2)) != hipSuccess ||
+            hipMalloc(&c->d_direct, std::max<size_t>(c->direct_blocks.size(), 1) * sizeof(int2)) != hipSuccess)
             return bail(fail(IMEX_ENOMEM, "halo table allocation failed"));
         if (hipMemcpy(c->d_segs, c->segs.data(), sb, hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(c->d_pack, c->pack_blocks.data(), c->pack_blocks.size() * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(c->d_unpack, c->unpack_blocks.data(), c->unpack_blocks.size() * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess)
+            hipMemcpy(c->d_unpack, c->unpack_blocks.data(), c->unpack_blocks.size() * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(c->d_direct, c->direct_blocks.data(), c->direct_blocks.size() * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess)
             return bail(fail(IMEX_EHIP, "halo table upload failed"));
     }
     const size_t bb = c->boxes.size() * sizeof(BoxDesc);
@@ -1008,6 +1076,9 @@ int imex_create(imex_ctx **out, const imex_config *cfg_in, const imex_domain *do
         return bail(fail(IMEX_ENOMEM, "descriptor allocation failed"));
     if (hipMemcpy(c->d_boxes, c->boxes.data(), bb, hipMemcpyHostToDevice) != hipSuccess)
         return bail(fail(IMEX_EHIP, "descriptor upload failed"));
+    if (hipMalloc(&c->d_pass, c->boxes.size() * sizeof(PassBox)) != hipSuccess)
+        return bail(fail(IMEX_ENOMEM, "descriptor allocation failed"));
+    if (c->two_step && (r = ts_prepare(c)) != IMEX_OK) return bail(r);
     r = upload_tiles(c);
     if (r != IMEX_OK) return bail(r);
 
@@ -1453,6 +1524,8 @@ int imex_query(imex_ctx *c, imex_info *info)
     info->two_step_zseg = c->ts_zseg;
     info->two_step_xcds = c->ts_xcds;
     info->two_step_prefetch = c->ts_pf;
+    info->halo_direct = c->tiled ? (c->halo_direct ? 1 : 0) : 0;
+    info->halo_depth = c->tiled ? c->halo_depth : 0;
     info->grid = c->tiled ? 1 : 0;
     info->rx = c->rgrid[0]; info->ry = c->rgrid[1]; info->rz = c->rgrid[2];
     return IMEX_OK;

@@ -53,7 +53,8 @@ struct HaloSeg {
     int ex, ey, ez, ncell;      // extents
     int s_sy, s_sz, d_sy, d_sz; // row / plane strides
     unsigned mask;              // directions carried (bit ii)
-    int pack;                   // 1: tile -> buffer, 0: buffer -> tile
+    int pack;                   // 1: tile -> buffer, 0: buffer -> tile,
+                                // 2: tile boundary -> neighbour's ghosts
 };
 
 enum { OP_COLLIDE = 0, OP_FUSED = 1, OP_STREAM = 2 };
@@ -456,9 +457,17 @@ __device__ inline void lds_pull_cell(const typename LdsWord<S>::type *gm, const 
 // iteration, so the HBM latency of the first step hides behind the second
 // (barriers wait for LDS traffic only).  Costs 27 registers per ring cell a
 // thread owns; used where the workgroup's LDS share leaves few waves per CU.
-template <class S, int TX, int TY, int NTH, bool NT, bool PF>
+// Workgroups of a two-step pass per box: the first one, the box and its
+// column grid (tiles_x columns across, ntile per z-segment).
+struct PassBox { int wg0, box, tiles_x, ntile; };
+
+// WRAP: periodic box, ring and z-planes wrap; otherwise the ring and the
+// planes z0 - 1 and z1 are ghost cells, filled two layers deep with all 27
+// directions before the pass.
+template <class S, bool WRAP, int TX, int TY, int NTH, bool NT, bool PF>
 __global__ void __launch_bounds__(NTH)
-k_step2(const S *__restrict__ src, S *__restrict__ dst, const BoxDesc *__restrict__ boxes, int xcds, int zseg)
+k_step2(const S *__restrict__ src, S *__restrict__ dst, const BoxDesc *__restrict__ boxes,
+        const PassBox *__restrict__ pb, int npb, int xcds, int zseg)
 {
     using R = typename Store<S>::real;
     using Wd = typename LdsWord<S>::type;
@@ -466,10 +475,13 @@ k_step2(const S *__restrict__ src, S *__restrict__ dst, const BoxDesc *__restric
     constexpr int NC = (RC + NTH - 1) / NTH;   // ring cells per thread
     __shared__ Wd lds[54 * RC];
     constexpr auto seq = std::make_integer_sequence<int, Q>{};
-    const BoxDesc &d = boxes[0];
-    const int tiles_x = d.nx / TX, ntile = tiles_x * (d.ny / TY);
     const int w = xcd_slot((int)blockIdx.x, (int)gridDim.x, xcds, 0);
-    const int tile = w % ntile, seg = w / ntile;
+    int bi = 0;
+    while (bi + 1 < npb && pb[bi + 1].wg0 <= w) ++bi;
+    const PassBox B = pb[bi];
+    const BoxDesc &d = boxes[B.box];
+    const int tiles_x = B.tiles_x, ntile = B.ntile;
+    const int tile = (w - B.wg0) % ntile, seg = (w - B.wg0) / ntile;
     const int x0 = (tile % tiles_x) * TX, y0 = (tile / tiles_x) * TY;
     const int z0 = seg * zseg, z1 = min(d.nz, z0 + zseg);
     const S *sbox = src + d.base + d.zorg;
@@ -478,14 +490,18 @@ k_step2(const S *__restrict__ src, S *__restrict__ dst, const BoxDesc *__restric
 
     // ring cell q of plane p: loads its pulled populations into f
     auto load_ring = [&](int p, int q, R (&f)[Q]) {
-        const int z = p < 0 ? p + d.nz : (p >= d.nz ? p - d.nz : p);
-        const int zm = z == 0 ? d.nz - 1 : z - 1, zp = z == d.nz - 1 ? 0 : z + 1;
+        int z = p, zm = p - 1, zp = p + 1;
+        int x = x0 - 1 + q % RX, y = y0 - 1 + q / RX;
+        if constexpr (WRAP) {
+            z = p < 0 ? p + d.nz : (p >= d.nz ? p - d.nz : p);
+            zm = z == 0 ? d.nz - 1 : z - 1;
+            zp = z == d.nz - 1 ? 0 : z + 1;
+            if (x < 0) x += d.nx; else if (x >= d.nx) x -= d.nx;
+            if (y < 0) y += d.ny; else if (y >= d.ny) y -= d.ny;
+        }
         const Planes<S, true> P(sbox + (long long)zm * d.sz, sbox + (long long)z * d.sz,
                                 sbox + (long long)zp * d.sz, dbox);
-        int x = x0 - 1 + q % RX, y = y0 - 1 + q / RX;
-        if (x < 0) x += d.nx; else if (x >= d.nx) x -= d.nx;
-        if (y < 0) y += d.ny; else if (y >= d.ny) y -= d.ny;
-        const Nbr n = neighbours<true>(d, x, y, z);
+        const Nbr n = neighbours<WRAP>(d, x, y, z);
         load_cell_io<S, true>(P, dsb, d.org_xy + x + y * d.sy, n, f, seq);
     };
     // collide ring cell q of the intermediate plane p and keep it in LDS
@@ -787,9 +803,12 @@ k_import(S *__restrict__ dst, const BoxDesc *__restrict__ boxes, int box,
 }
 
 // ---------------------------------------------------------------------------
-// Halo pack / unpack over every segment of every tile in one launch.  Each
-// workgroup copies up to blockDim cells of one segment for all of its
-// directions.  Copies are bit-exact moves of the stored format.
+// Halo pack / unpack over every segment of every tile in one launch, or the
+// direct fill (pack == 2: a tile's boundary slab straight into the
+// neighbour's ghost slab of the same pool; the slabs read are interior cells,
+// those written ghost cells, so no cell is both).  Each workgroup copies up
+// to blockDim cells of one segment for all of its directions.  Copies are
+// bit-exact moves of the stored format.
 // ---------------------------------------------------------------------------
 template <class S>
 __global__ void __launch_bounds__(256)
@@ -811,7 +830,7 @@ k_halo(const S *__restrict__ src, S *__restrict__ dst, const HaloSeg *__restrict
         for (int ii = 0; ii < Q; ++ii) {
             if (sg.mask & (1u << ii)) {
                 const long long sdir = sg.pack ? ii : slot;
-                const long long ddir = sg.pack ? slot : ii;
+                const long long ddir = sg.pack == 1 ? slot : ii;
                 dst[dofs + ddir * sg.d_ds] = src[so + sdir * sg.s_ds];
                 ++slot;
             }

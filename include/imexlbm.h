@@ -110,6 +110,12 @@ typedef struct imex_config {
     int two_step_prefetch; /* load the next plane's populations into
                              registers ahead of the barrier; 1 = on, 0 =
                              off, -1 = auto (on for 8- and 4-byte formats) */
+    int halo_direct;      /* IMEX_TILED: 1 = each tile's boundary slabs are
+                             copied straight into its neighbours' ghosts
+                             (one launch), 0 = packed into an exchange
+                             buffer and unpacked from it (two launches:
+                             the shape of a cross-GPU exchange); -1 = auto
+                             (1)                                         */
 } imex_config;
 
 typedef struct imex_info {
@@ -130,6 +136,9 @@ typedef struct imex_info {
     int xcd_run;
     int two_step;             /* 1 if pairs of steps run as one pass      */
     int two_step_tile, two_step_zseg, two_step_xcds, two_step_prefetch;
+    int halo_direct;          /* IMEX_TILED: 1 direct ghost fill, 0 pack/unpack */
+    int halo_depth;           /* IMEX_TILED: ghost layers filled (2 when two
+                                 steps run per pass)                          */
 } imex_info;
 
 /* Defaults for every field of imex_config: FP64, wrap, 256-thread

@@ -144,19 +144,22 @@ def test_interleaved_api_calls_keep_reference_semantics():
     o.close()
 
 
+@pytest.mark.parametrize("direct", [1, 0])
 @pytest.mark.parametrize("ntiles,depth,n", [(8, 1, 16), (8, 3, 16), (2, 1, 17), (4, 2, 17),
                                             (12, 1, 18)])
-def test_tiled_matches_wrap_bitwise(ntiles, depth, n):
+def test_tiled_matches_wrap_bitwise(ntiles, depth, n, direct):
+    # direct: boundary slabs copied straight into the neighbours' ghosts;
+    # otherwise packed into the exchange buffer and unpacked from it
     from imexlbm_amd import capi
     d = _dom(n, re=400)
     with _ctx(d, precision="fp64") as w:
         w.initialize(); w.step(12)
         fw, cw = w.get_f(), w.conserved()
-    with _ctx(d, precision="fp64", mode=capi.TILED, ntiles=ntiles, halo_depth=depth) as t:
+    with _ctx(d, precision="fp64", mode=capi.TILED, ntiles=ntiles, halo_depth=depth, halo_direct=direct) as t:
         t.initialize(); t.step(12)
         ft, ct = t.get_f(), t.conserved()
         info = t.info()
-    assert info["ntiles"] == ntiles
+    assert info["ntiles"] == ntiles and info["halo_direct"] == direct and info["halo_depth"] == depth
     assert np.array_equal(fw, ft)
     assert abs(cw[0] - ct[0]) / cw[0] <= 1e-13 and abs(cw[4] - ct[4]) / cw[4] <= 1e-13
 
@@ -275,18 +278,22 @@ def test_get_macro_matches_moments_of_oracle_state():
         assert np.abs(got - want).max() < 1e-15
 
 
-def test_tiled_256_three_layer_halo_matches_wrap():
-    # the 256^3 deck's decomposition check: 2x2x2 tiles, 3-layer pack/unpack
+@pytest.mark.parametrize("direct,two", [(1, -1), (0, 0)])
+def test_tiled_256_three_layer_halo_matches_wrap(direct, two):
+    # the 256^3 deck's decomposition check: 2x2x2 tiles, 3-layer ghost fill
+    # (direct with two steps per pass, or pack/unpack one step per pass)
     from imexlbm_amd import capi
     d = _dom(256, re=1600)
     with _ctx(d, precision="fp64") as w:
         w.initialize(); w.step(6)
         cw = w.conserved()
         fw = w.get_f()
-    with _ctx(d, precision="fp64", mode=capi.TILED, ntiles=8, halo_depth=3) as t:
+    with _ctx(d, precision="fp64", mode=capi.TILED, ntiles=8, halo_depth=3, halo_direct=direct,
+              two_step=two) as t:
         t.initialize(); t.step(6)
         ct = t.conserved()
-        assert t.info()["ntiles"] == 8
+        info = t.info()
+        assert info["ntiles"] == 8 and info["two_step"] == (1 if two else 0)
         assert np.array_equal(fw, t.get_f())
     del fw
     assert abs(cw[0] - ct[0]) / cw[0] <= 1e-13 and abs(cw[4] - ct[4]) / cw[4] <= 1e-13

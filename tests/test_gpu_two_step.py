@@ -144,10 +144,41 @@ def test_two_step_eligibility():
         _ctx(d, precision="fp32", two_step=1)
     with pytest.raises(capi.ImexError):
         _ctx(_dom(64), precision="fp64", two_step=1, two_step_tile=0)   # FP64 builds only 32x8
-    with pytest.raises(capi.ImexError):
-        _ctx(_dom(64), precision="fp32", two_step=1, mode=capi.TILED, ntiles=2)
-    with _ctx(_dom(64), precision="fp32", mode=capi.TILED, ntiles=2) as c:
+    with pytest.raises(capi.ImexError):      # tiles of 24 x 40 cells: no column tile divides them
+        _ctx(_dom(48, ny=40, nz=8), precision="fp32", two_step=1, mode=capi.TILED, ntiles=2)
+    with pytest.raises(capi.ImexError):      # batches keep one step per pass
+        _ctx([_dom(64), _dom(64)], precision="fp32", two_step=1)
+    with _ctx(_dom(64, nz=1), precision="fp32", mode=capi.TILED, ntiles=2) as c:   # tiles one plane deep
         assert c.info()["two_step"] == 0
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "fp16", "bf16"])
+@pytest.mark.parametrize("ntiles,halo,direct", [(2, 1, 1), (4, 2, 0), (8, 3, 1), (8, 1, 0)])
+def test_tiled_two_step_equals_wrap_single_steps(prec, ntiles, halo, direct):
+    # tiles exchange two ghost layers with all 27 directions once per pass;
+    # populations must equal the periodic single-box run bit for bit
+    from imexlbm_amd import capi
+    d = _dom(64, ny=64, nz=24)
+    with _ctx(d, precision=prec, mode=capi.TILED, ntiles=ntiles, halo_depth=halo, two_step=1,
+              halo_direct=direct) as a:
+        a.initialize()
+        a.step(7)
+        info = a.info()
+        assert info["two_step"] == 1 and info["halo_depth"] == max(2, halo) and info["halo_direct"] == direct
+        fa, ca = a.get_f(), a.conserved()
+        a.collision(); a.streaming(); a.collision(); a.streaming()
+        a.collision(); a.streaming()
+        fa2 = a.get_f()
+    with _ctx(d, precision=prec, two_step=0) as b:
+        b.initialize()
+        b.step(7)
+        fb, cb = b.get_f(), b.conserved()
+        b.step(3)
+        fb2 = b.get_f()
+    assert np.array_equal(fa, fb)
+    assert np.array_equal(fa2, fb2)
+    for k in (0, 4):
+        assert abs(ca[k] - cb[k]) <= 1e-12 * abs(cb[k])
 
 
 def test_two_step_full_size_fp16_512():
