@@ -3,9 +3,9 @@
 64 independent periodic TGV instances whose cube edge is drawn from
 {32, 48, 64, 96, 128}, Re log-uniform in [100, 3200] and u0 uniform in
 [0.02, 0.08] (numpy default_rng, fixed seed) advance together: one work-tile
-table covers every instance, each step is one launch, the steps are replayed
-from a captured hipGraph, and the launch shape comes from the library's
-autotuner.  Reported beside it:
+table (or, two steps per pass, one column-tile table) covers every instance,
+each step or pair of steps is one launch, replayed from a captured hipGraph,
+and the launch shape comes from the library's autotuner.  Reported beside it:
 
   sequential_launch   the same context, one launch per instance per step
   one_after_another   each instance in its own context, run to completion in turn
@@ -63,7 +63,8 @@ def run(count=64, steps=1000, prec="fp32", log=sys.stderr):
         res["autotune_ms"] = c.autotune(reps=5)
         info = c.info()
         res["launch"] = {k: info[k] for k in ("block", "cells_per_thread", "persistent", "nontemporal",
-                                             "xcd_groups", "work_tiles")}
+                                             "xcd_groups", "work_tiles", "two_step", "two_step_tile",
+                                             "two_step_zseg", "two_step_xcds", "two_step_prefetch")}
         wall, kern = _timed(c, steps)
         res["batched_mlups"] = cells * steps / (wall / 1e3) / 1e6
         res["batched_ms_per_step"] = wall / steps

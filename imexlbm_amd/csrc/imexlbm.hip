@@ -497,19 +497,36 @@ void ts_dims(int tile, int &tx, int &ty)
     ty = (tile == 1 || tile == 4) ? 16 : 8;
 }
 
-// One domain (one periodic box, or the tiles of a tiled domain, at least two
-// planes deep) whose boxes' x and y extents the column tile divides; FP64
-// builds only the 32x8 tiles (its LDS ring needs 147 KB).
+// Periodic boxes (one domain or a batch, at least one column tile across;
+// the last column may be ragged) or the tiles of one tiled domain (divided
+// exactly by the column tile, at least two planes deep), in one-launch mode;
+// FP64 builds only the 32x8 tiles (its LDS ring needs 147 KB).
 bool ts_allowed(const imex_ctx *c, int tile)
 {
-    if (c->inst.size() != 1 || tile < 0 || tile > 4) return false;
+    if (tile < 0 || tile > 4 || (c->tiled && c->inst.size() != 1)) return false;
+    if (c->inst.size() > 1 && c->cfg.sequential) return false;
     if (c->ssize == 8 && (tile < 2 || tile == 4)) return false;
     if (c->ssize == 4 && tile == 4) return false;
     int tx, ty;
     ts_dims(tile, tx, ty);
-    for (const BoxDesc &b : c->boxes)
-        if (b.nx % tx || b.ny % ty || (c->tiled && b.nz < 2)) return false;
+    for (const BoxDesc &b : c->boxes) {
+        if (c->tiled ? (b.nx % tx || b.ny % ty || b.nz < 2) : (b.nx < tx || b.ny < ty)) return false;
+    }
     return true;
+}
+
+// Cells the pass's column tiles cover over the cells of the boxes (1 when
+// the tile divides every box).
+double ts_cover(const imex_ctx *c, int tile)
+{
+    int tx, ty;
+    ts_dims(tile, tx, ty);
+    double cov = 0, cells = 0;
+    for (const BoxDesc &b : c->boxes) {
+        cov += (double)((b.nx + tx - 1) / tx * tx) * ((b.ny + ty - 1) / ty * ty) * b.nz;
+        cells += (double)b.nx * b.ny * b.nz;
+    }
+    return cells > 0 ? cov / cells : 1e30;
 }
 
 // Per-box workgroup table of the two-step pass for the current tile and
@@ -526,7 +543,7 @@ int ts_prepare(imex_ctx *c)
     for (int bi = 0; bi < (int)c->boxes.size(); ++bi) {
         const BoxDesc &b = c->boxes[bi];
         const int zseg = std::max(1, std::min(c->ts_zseg, b.nz));
-        const int tiles_x = b.nx / tx, ntile = tiles_x * (b.ny / ty);
+        const int tiles_x = (b.nx + tx - 1) / tx, ntile = tiles_x * ((b.ny + ty - 1) / ty);
         t.push_back({(int)wg, bi, tiles_x, ntile});
         wg += (long long)ntile * ((b.nz + zseg - 1) / zseg);
     }
@@ -538,16 +555,32 @@ int ts_prepare(imex_ctx *c)
     return IMEX_OK;
 }
 
+// Workgroups of a two-step pass with z-sweeps of zseg planes.
+long long ts_grid(const imex_ctx *c, int tile, int zseg)
+{
+    int tx, ty;
+    ts_dims(tile, tx, ty);
+    long long g = 0;
+    for (const BoxDesc &b : c->boxes)
+        g += (long long)((b.nx + tx - 1) / tx) * ((b.ny + ty - 1) / ty) * ((b.nz + zseg - 1) / zseg);
+    return g;
+}
+
 // Planes per z-sweep: more sweeps give more workgroups, and with the tile
 // order by XCD each XCD a z-range of its own, but every sweep recomputes two
 // intermediate planes.  Autotuned optima at 256^3 and 512^3 lie near nz/8
 // (at least 32) for 2-byte formats and near nz/4 (at least 64) for the
-// others, whose passes carry more work per plane.
-int ts_default_zseg(const imex_ctx *c, int /*tile*/)
+// others, whose passes carry more work per plane; smaller boxes shorten the
+// sweeps (not below 16 planes) until the pass has two workgroups per CU.
+// Returns 0 when even then it would not have one per CU: such a small domain
+// runs faster one step per pass.
+int ts_default_zseg(const imex_ctx *c, int tile)
 {
-    const int nz = c->boxes[0].nz;
-    const int z = c->ssize == 2 ? std::max(32, nz / 8) : std::max(64, nz / 4);
-    return std::min(z, nz);
+    int nz = 1;
+    for (const BoxDesc &b : c->boxes) nz = std::max(nz, b.nz);
+    int z = std::min(nz, c->ssize == 2 ? std::max(32, nz / 8) : std::max(64, nz / 4));
+    while (z > 16 && ts_grid(c, tile, z) < 2LL * c->ncu) z = std::max(16, z / 2);
+    return ts_grid(c, tile, z) >= c->ncu ? z : 0;
 }
 
 template <class S, int TX, int TY, int NTH>
@@ -1018,23 +1051,27 @@ int imex_create(imex_ctx **out, const imex_config *cfg_in, const imex_domain *do
     if (r != IMEX_OK) return bail(r);
     build_tiles(c);
     {
-        const int def_tile = c->ssize == 8 ? 3 : 0;
-        c->ts_tile = cfg.two_step_tile >= 0 ? cfg.two_step_tile : def_tile;
-        if (!ts_allowed(c, c->ts_tile) && cfg.two_step_tile < 0)
+        // default tile: the fewest covered cells, ties in this order
+        c->ts_tile = cfg.two_step_tile >= 0 ? cfg.two_step_tile : (c->ssize == 8 ? 3 : 0);
+        if (cfg.two_step_tile < 0) {
+            double best = 1e30;
             for (int t : {0, 1, 3, 2, 4})
-                if (ts_allowed(c, t)) { c->ts_tile = t; break; }
+                if (ts_allowed(c, t) && ts_cover(c, t) < best - 1e-9) { best = ts_cover(c, t); c->ts_tile = t; }
+        }
         const bool ok = ts_allowed(c, c->ts_tile);
+        const int zdef = ok ? ts_default_zseg(c, c->ts_tile) : 0;
         if (cfg.two_step == 1 && !ok)
-            return bail(fail(IMEX_EINVAL, "two_step needs one domain whose boxes' nx and ny the column tile "
-                                          "divides (tiled: boxes at least 2 planes deep; FP64: 32x8 tiles only)"));
-        c->two_step = (cfg.two_step != 0 && ok) ? 1 : 0;
+            return bail(fail(IMEX_EINVAL, "two_step needs boxes whose nx and ny the column tile divides (tiled: "
+                                          "boxes at least 2 planes deep; batches: one launch for all; FP64: 32x8 "
+                                          "tiles only)"));
+        c->two_step = ok && (cfg.two_step == 1 || (cfg.two_step < 0 && zdef > 0)) ? 1 : 0;
         if (c->two_step && c->tiled) {
             // the second step pulls through the first step's ring: two ghost
             // layers, with every direction of the first one
             c->halo_all = true;
             c->halo_depth = std::max(c->halo_depth, 2);
         }
-        c->ts_zseg = cfg.two_step_zseg > 0 ? cfg.two_step_zseg : (ok ? ts_default_zseg(c, c->ts_tile) : 1);
+        c->ts_zseg = cfg.two_step_zseg > 0 ? cfg.two_step_zseg : (zdef > 0 ? zdef : 16);
         c->ts_xcds = cfg.two_step_xcds > 0 ? cfg.two_step_xcds : 8;
         c->ts_pf = cfg.two_step_prefetch == 1 || (cfg.two_step_prefetch < 0 && c->ssize >= 4) ? 1 : 0;
     }
@@ -1380,9 +1417,11 @@ int imex_autotune(imex_ctx *c, int reps, double *best_ms)
         const int pf_min = c->cfg.two_step_prefetch == 1 ? 1 : 0;
         for (int tile : {0, 1, 2, 3, 4}) {
             if (!ts_allowed(c, tile)) continue;
-            std::vector<int> zs = {ts_default_zseg(c, tile)};
+            std::vector<int> zs;
+            if (const int z = ts_default_zseg(c, tile)) zs.push_back(z);
             for (int div : {1, 2, 4, 8, 16})
-                if (nz / div >= 16 && std::find(zs.begin(), zs.end(), nz / div) == zs.end()) zs.push_back(nz / div);
+                if (nz / div >= std::min(16, nz) && std::find(zs.begin(), zs.end(), nz / div) == zs.end())
+                    zs.push_back(nz / div);
             for (int z : zs)
                 for (int x : {1, 8})
                     for (int pf = pf_min; pf <= pf_max; ++pf) c2.push_back({tile, z, x, pf});

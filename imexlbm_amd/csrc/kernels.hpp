@@ -461,9 +461,11 @@ __device__ inline void lds_pull_cell(const typename LdsWord<S>::type *gm, const 
 // column grid (tiles_x columns across, ntile per z-segment).
 struct PassBox { int wg0, box, tiles_x, ntile; };
 
-// WRAP: periodic box, ring and z-planes wrap; otherwise the ring and the
-// planes z0 - 1 and z1 are ghost cells, filled two layers deep with all 27
-// directions before the pass.
+// WRAP: periodic box, ring and z-planes wrap, and the last column tile may
+// run past nx or ny (TX <= nx, TY <= ny): its ring wraps onto real cells and
+// its output cells beyond the box are not stored.  Otherwise the tile divides
+// the box, and the ring and the planes z0 - 1 and z1 are ghost cells, filled
+// two layers deep with all 27 directions before the pass.
 template <class S, bool WRAP, int TX, int TY, int NTH, bool NT, bool PF>
 __global__ void __launch_bounds__(NTH)
 k_step2(const S *__restrict__ src, S *__restrict__ dst, const BoxDesc *__restrict__ boxes,
@@ -526,6 +528,7 @@ k_step2(const S *__restrict__ src, S *__restrict__ dst, const BoxDesc *__restric
         const Planes<S, true> P(wp, wp, wp, wp);
         for (int q = (int)threadIdx.x; q < IC; q += NTH) {
             const int tx = q % TX, ty = q / TX;
+            if (WRAP && (x0 + tx >= d.nx || y0 + ty >= d.ny)) continue;   // past a ragged edge
             R f[Q];
             lds_pull_cell<S, RC, RX>(gm, g0, gp, (ty + 1) * RX + tx + 1, f, seq);
             if constexpr (Store<S>::shifted) collide32(f, d.omega_f);

@@ -75,7 +75,7 @@ def test_two_step_tile_order(prec):
 def test_two_step_fp64_matches_oracle_bitwise():
     from oracle import oracle
     d = _dom(32, re=1600)
-    with _ctx(d, precision="fp64") as c:      # auto: on for 32^3 FP64
+    with _ctx(d, precision="fp64", two_step=1) as c:
         c.initialize()
         assert c.info()["two_step"] == 1
         c.step(40)
@@ -137,17 +137,24 @@ def test_autotune_with_two_step_candidates(prec):
 
 def test_two_step_eligibility():
     from imexlbm_amd import capi
-    d = _dom(48, ny=40, nz=8)
-    with _ctx(d, precision="fp32") as c:          # 48 is no multiple of 32: auto stays off
+    with _ctx(_dom(32), precision="fp64") as c:   # auto: 4 column tiles are too few workgroups
         assert c.info()["two_step"] == 0
-    with pytest.raises(capi.ImexError):
-        _ctx(d, precision="fp32", two_step=1)
+    with _ctx(_dom(256, nz=64), precision="fp16") as c:   # 128 tiles x 4 sweeps of 16 planes
+        info = c.info()
+        assert info["two_step"] == 1 and info["two_step_zseg"] == 16
+    d = _dom(48, ny=40, nz=8)
+    with _ctx(d, precision="fp32") as c:          # 6 column tiles: too few workgroups, auto stays off
+        assert c.info()["two_step"] == 0
+    with pytest.raises(capi.ImexError):           # narrower than every column tile
+        _ctx(_dom(24, ny=40, nz=8), precision="fp32", two_step=1)
     with pytest.raises(capi.ImexError):
         _ctx(_dom(64), precision="fp64", two_step=1, two_step_tile=0)   # FP64 builds only 32x8
     with pytest.raises(capi.ImexError):      # tiles of 24 x 40 cells: no column tile divides them
         _ctx(_dom(48, ny=40, nz=8), precision="fp32", two_step=1, mode=capi.TILED, ntiles=2)
-    with pytest.raises(capi.ImexError):      # batches keep one step per pass
-        _ctx([_dom(64), _dom(64)], precision="fp32", two_step=1)
+    with pytest.raises(capi.ImexError):      # a batch member of 16 cells across
+        _ctx([_dom(64), _dom(16, ny=64)], precision="fp32", two_step=1)
+    with pytest.raises(capi.ImexError):      # one launch per instance: no pass table
+        _ctx([_dom(64), _dom(64)], precision="fp32", two_step=1, sequential=1)
     with _ctx(_dom(64, nz=1), precision="fp32", mode=capi.TILED, ntiles=2) as c:   # tiles one plane deep
         assert c.info()["two_step"] == 0
 
@@ -201,3 +208,38 @@ def test_two_step_full_size_fp16_512():
             del m
     assert np.array_equal(out[0], out[2])
     assert np.array_equal(out[1], out[3])
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp16"])
+def test_batched_two_step_equals_instances_alone(prec):
+    # three periodic instances of different sizes, Re and rho0 in one pass
+    # per two steps; each must equal its own single-instance one-step run
+    doms = [_dom(64, ny=32, nz=16, re=300), _dom(32, ny=64, nz=8, re=900, rho0=1.1),
+            _dom(96, ny=16, nz=5, re=150, u0=0.03)]
+    with _ctx(doms, precision=prec, two_step=1) as a:
+        a.initialize()
+        a.step(9)
+        assert a.info()["two_step"] == 1
+        got = [a.get_f(k) for k in range(3)]
+    for k, d in enumerate(doms):
+        with _ctx(d, precision=prec, two_step=0) as b:
+            b.initialize()
+            b.step(9)
+            assert np.array_equal(got[k], b.get_f())
+
+
+RAGGED = [("fp64", 3), ("fp64", 2), ("fp32", 0), ("fp32", 1), ("fp16", 0), ("fp16", 3), ("bf16", 1)]
+
+
+@pytest.mark.parametrize("prec,tile", RAGGED)
+@pytest.mark.parametrize("shape", [(100, 40, 12), (48, 72, 7), (96, 50, 5), (33, 17, 4)])
+def test_two_step_ragged_column_tiles(prec, tile, shape):
+    # the last column tile runs past nx / ny: its ring wraps onto real cells
+    # and its cells beyond the box are not stored
+    tx, ty = {0: (64, 8), 1: (32, 16), 2: (32, 8), 3: (32, 8)}[tile]
+    if shape[0] < tx or shape[1] < ty:
+        pytest.skip("box narrower than the tile")
+    d = _dom(*shape)
+    fa, fb, ca, cb = _pair(d, prec, 7, two_step_tile=tile, two_step_zseg=3)
+    assert np.array_equal(fa, fb)
+    assert np.array_equal(np.array(ca), np.array(cb))
