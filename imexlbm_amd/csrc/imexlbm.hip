@@ -154,7 +154,7 @@ struct imex_ctx {
     int io16 = 0;
     int xcd_run = 0;
     // two steps per pass: on/off, column tile, z-planes per sweep, tile order
-    int two_step = 0, ts_tile = 0, ts_zseg = 1, ts_xcds = 1, ts_pf = 0;
+    int two_step = 0, ts_tile = 0, ts_zseg = 1, ts_xcds = 1, ts_pf = 0, ts_buf = 0;
     bool halo_all = false;       // tiled two-step: ghosts carry all 27 directions
     bool halo_direct = true;     // tiled: one direct copy instead of pack + unpack
     long long direct_bytes = 0;  // bytes the direct fill moves
@@ -568,18 +568,20 @@ long long ts_grid(const imex_ctx *c, int tile, int zseg)
 
 // Planes per z-sweep: more sweeps give more workgroups, and with the tile
 // order by XCD each XCD a z-range of its own, but every sweep recomputes two
-// intermediate planes.  Autotuned optima at 256^3 and 512^3 lie near nz/8
-// (at least 32) for 2-byte formats and near nz/4 (at least 64) for the
-// others, whose passes carry more work per plane; smaller boxes shorten the
-// sweeps (not below 16 planes) until the pass has two workgroups per CU.
-// Returns 0 when even then it would not have one per CU: such a small domain
-// runs faster one step per pass.
+// intermediate planes.  Autotuned optima at 256^3 and 512^3: the 64x16 tile
+// (one 128 KB workgroup per CU) sweeps as far as one workgroup per CU
+// allows; otherwise near nz/8 (at least 32) for 2-byte formats and nz/4 (at
+// least 64) for the others, whose passes carry more work per plane, halved
+// (not below 16 planes) until the pass has two workgroups per CU.  Returns 0
+// when the pass would not have one workgroup per CU even then: such a small
+// domain runs faster one step per pass.
 int ts_default_zseg(const imex_ctx *c, int tile)
 {
     int nz = 1;
     for (const BoxDesc &b : c->boxes) nz = std::max(nz, b.nz);
-    int z = std::min(nz, c->ssize == 2 ? std::max(32, nz / 8) : std::max(64, nz / 4));
-    while (z > 16 && ts_grid(c, tile, z) < 2LL * c->ncu) z = std::max(16, z / 2);
+    const long long want = (tile == 4 ? 1LL : 2LL) * c->ncu;
+    int z = tile == 4 ? nz : std::min(nz, c->ssize == 2 ? std::max(32, nz / 8) : std::max(64, nz / 4));
+    while (z > 16 && ts_grid(c, tile, z) < want) z = std::max(16, z / 2);
     return ts_grid(c, tile, z) >= c->ncu ? z : 0;
 }
 
@@ -591,12 +593,14 @@ int launch_ts(imex_ctx *c, int src, int dst)
     const S *sp = (const S *)pool_ptr(c, src);
     S *dp = (S *)pool_ptr(c, dst);
     const int npb = (int)c->boxes.size(), zseg = std::max(1, c->ts_zseg);
-#define TS(W, P) hipLaunchKernelGGL((k_step2<S, W, TX, TY, NTH, true, P>), g, b, 0, c->stream, sp, dp, c->d_boxes, \
-                                    c->d_pass, npb, c->ts_xcds, zseg)
+#define TS(W, P, G) hipLaunchKernelGGL((k_step2<S, W, TX, TY, NTH, true, P, G>), g, b, 0, c->stream, sp, dp, \
+                                       c->d_boxes, c->d_pass, npb, c->ts_xcds, zseg)
     if (c->tiled) {
-        if (c->ts_pf) TS(false, true); else TS(false, false);
+        if (c->ts_pf) TS(false, true, true); else TS(false, false, true);
+    } else if (c->ts_buf) {
+        if (c->ts_pf) TS(true, true, false); else TS(true, false, false);
     } else {
-        if (c->ts_pf) TS(true, true); else TS(true, false);
+        if (c->ts_pf) TS(true, true, true); else TS(true, false, true);
     }
 #undef TS
     HIPC(hipGetLastError());
@@ -932,6 +936,7 @@ void imex_config_default(imex_config *cfg)
     cfg->two_step = -1;
     cfg->two_step_tile = -1;
     cfg->two_step_prefetch = -1;
+    cfg->two_step_io = -1;
     cfg->halo_direct = -1;
 }
 
@@ -1052,10 +1057,10 @@ int imex_create(imex_ctx **out, const imex_config *cfg_in, const imex_domain *do
     build_tiles(c);
     {
         // default tile: the fewest covered cells, ties in this order
-        c->ts_tile = cfg.two_step_tile >= 0 ? cfg.two_step_tile : (c->ssize == 8 ? 3 : 0);
+        c->ts_tile = cfg.two_step_tile >= 0 ? cfg.two_step_tile : (c->ssize == 8 ? 3 : (c->ssize == 2 ? 4 : 0));
         if (cfg.two_step_tile < 0) {
             double best = 1e30;
-            for (int t : {0, 1, 3, 2, 4})
+            for (int t : {4, 0, 1, 3, 2})
                 if (ts_allowed(c, t) && ts_cover(c, t) < best - 1e-9) { best = ts_cover(c, t); c->ts_tile = t; }
         }
         const bool ok = ts_allowed(c, c->ts_tile);
@@ -1074,6 +1079,7 @@ int imex_create(imex_ctx **out, const imex_config *cfg_in, const imex_domain *do
         c->ts_zseg = cfg.two_step_zseg > 0 ? cfg.two_step_zseg : (zdef > 0 ? zdef : 16);
         c->ts_xcds = cfg.two_step_xcds > 0 ? cfg.two_step_xcds : 8;
         c->ts_pf = cfg.two_step_prefetch == 1 || (cfg.two_step_prefetch < 0 && c->ssize >= 4) ? 1 : 0;
+        c->ts_buf = cfg.two_step_io == 0 && !c->tiled ? 1 : 0;
     }
     if (c->tiled) build_halo(c);
 
@@ -1410,11 +1416,13 @@ int imex_autotune(imex_ctx *c, int reps, double *best_ms)
     // Two steps per pass: column tile x z-planes per sweep x tile order,
     // timed per pass (two steps) against two single-step launches.
     if (c->cfg.two_step != 0) {
-        struct Cand2 { int tile, zseg, xcds, pf; };
+        struct Cand2 { int tile, zseg, xcds, pf, buf; };
         std::vector<Cand2> c2;
         const int nz = c->boxes.empty() ? 0 : c->boxes[0].nz;
         const int pf_max = c->cfg.two_step_prefetch != 0 ? 1 : 0;
         const int pf_min = c->cfg.two_step_prefetch == 1 ? 1 : 0;
+        const int buf_max = (!c->tiled && c->cfg.two_step_io != 1) ? 1 : 0;
+        const int buf_min = (!c->tiled && c->cfg.two_step_io == 0) ? 1 : 0;
         for (int tile : {0, 1, 2, 3, 4}) {
             if (!ts_allowed(c, tile)) continue;
             std::vector<int> zs;
@@ -1424,13 +1432,15 @@ int imex_autotune(imex_ctx *c, int reps, double *best_ms)
                     zs.push_back(nz / div);
             for (int z : zs)
                 for (int x : {1, 8})
-                    for (int pf = pf_min; pf <= pf_max; ++pf) c2.push_back({tile, z, x, pf});
+                    for (int pf = pf_min; pf <= pf_max; ++pf)
+                        for (int buf = buf_min; buf <= buf_max; ++buf) c2.push_back({tile, z, x, pf, buf});
         }
-        const int keep[5] = {c->two_step, c->ts_tile, c->ts_zseg, c->ts_xcds, c->ts_pf};
+        const int keep[6] = {c->two_step, c->ts_tile, c->ts_zseg, c->ts_xcds, c->ts_pf, c->ts_buf};
         double best2 = 1e300;
         int pick = -1;
         for (size_t k = 0; k < c2.size(); ++k) {
             c->ts_tile = c2[k].tile; c->ts_zseg = c2[k].zseg; c->ts_xcds = c2[k].xcds; c->ts_pf = c2[k].pf;
+            c->ts_buf = c2[k].buf;
             TRY(launch_step2(c, c->cur, 1 - c->cur));
             HIPC(hipEventRecord(c->ev[0], c->stream));
             for (int r = 0; r < reps; ++r) TRY(launch_step2(c, c->cur, 1 - c->cur));
@@ -1440,8 +1450,8 @@ int imex_autotune(imex_ctx *c, int reps, double *best_ms)
             HIPC(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
             const double per_step = ms / reps / 2.0;
             if (std::getenv("IMEXLBM_AUTOTUNE_LOG"))
-                std::fprintf(stderr, "autotune two-step tile=%d zseg=%d xcds=%d pf=%d: %.4f ms per step\n",
-                             c2[k].tile, c2[k].zseg, c2[k].xcds, c2[k].pf, per_step);
+                std::fprintf(stderr, "autotune two-step tile=%d zseg=%d xcds=%d pf=%d buf=%d: %.4f ms per step\n",
+                             c2[k].tile, c2[k].zseg, c2[k].xcds, c2[k].pf, c2[k].buf, per_step);
             if (per_step < best2) { best2 = per_step; pick = (int)k; }
         }
         if (pick >= 0 && (best2 < best || c->cfg.two_step == 1)) {
@@ -1449,10 +1459,12 @@ int imex_autotune(imex_ctx *c, int reps, double *best_ms)
             c->two_step = 1;
             c->ts_tile = c2[pick].tile; c->ts_zseg = c2[pick].zseg; c->ts_xcds = c2[pick].xcds;
             c->ts_pf = c2[pick].pf;
+            c->ts_buf = c2[pick].buf;
             best = std::min(best, best2);
         } else {
             c->two_step = pick >= 0 ? 0 : keep[0];
             c->ts_tile = keep[1]; c->ts_zseg = keep[2]; c->ts_xcds = keep[3]; c->ts_pf = keep[4];
+            c->ts_buf = keep[5];
         }
     }
     drop_graphs(c);
@@ -1563,6 +1575,7 @@ int imex_query(imex_ctx *c, imex_info *info)
     info->two_step_zseg = c->ts_zseg;
     info->two_step_xcds = c->ts_xcds;
     info->two_step_prefetch = c->ts_pf;
+    info->two_step_io = c->ts_buf ? 0 : 1;
     info->halo_direct = c->tiled ? (c->halo_direct ? 1 : 0) : 0;
     info->halo_depth = c->tiled ? c->halo_depth : 0;
     info->grid = c->tiled ? 1 : 0;

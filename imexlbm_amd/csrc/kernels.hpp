@@ -466,7 +466,7 @@ struct PassBox { int wg0, box, tiles_x, ntile; };
 // its output cells beyond the box are not stored.  Otherwise the tile divides
 // the box, and the ring and the planes z0 - 1 and z1 are ghost cells, filled
 // two layers deep with all 27 directions before the pass.
-template <class S, bool WRAP, int TX, int TY, int NTH, bool NT, bool PF>
+template <class S, bool WRAP, int TX, int TY, int NTH, bool NT, bool PF, bool GIO = true>
 __global__ void __launch_bounds__(NTH)
 k_step2(const S *__restrict__ src, S *__restrict__ dst, const BoxDesc *__restrict__ boxes,
         const PassBox *__restrict__ pb, int npb, int xcds, int zseg)
@@ -501,8 +501,8 @@ k_step2(const S *__restrict__ src, S *__restrict__ dst, const BoxDesc *__restric
             if (x < 0) x += d.nx; else if (x >= d.nx) x -= d.nx;
             if (y < 0) y += d.ny; else if (y >= d.ny) y -= d.ny;
         }
-        const Planes<S, true> P(sbox + (long long)zm * d.sz, sbox + (long long)z * d.sz,
-                                sbox + (long long)zp * d.sz, dbox);
+        const Planes<S, GIO> P(sbox + (long long)zm * d.sz, sbox + (long long)z * d.sz,
+                               sbox + (long long)zp * d.sz, dbox);
         const Nbr n = neighbours<WRAP>(d, x, y, z);
         load_cell_io<S, true>(P, dsb, d.org_xy + x + y * d.sy, n, f, seq);
     };
@@ -525,7 +525,7 @@ k_step2(const S *__restrict__ src, S *__restrict__ dst, const BoxDesc *__restric
     auto second = [&](int k) {
         const Wd *gm = lds, *g0 = lds + (9 + 9 * ((k + 4) & 1)) * RC, *gp = lds + (27 + 9 * ((k + 2) % 3)) * RC;
         S *wp = dbox + (long long)k * d.sz;
-        const Planes<S, true> P(wp, wp, wp, wp);
+        const Planes<S, GIO> P(wp, wp, wp, wp);
         for (int q = (int)threadIdx.x; q < IC; q += NTH) {
             const int tx = q % TX, ty = q / TX;
             if (WRAP && (x0 + tx >= d.nx || y0 + ty >= d.ny)) continue;   // past a ragged edge

@@ -101,8 +101,9 @@ typedef struct imex_config {
     int two_step_tile;    /* column tile: 0 = 64x8 cells (512 threads),
                              1 = 32x16 (512), 2 = 32x8 (256), 3 = 32x8
                              (384), 4 = 64x16 (1024, 2-byte formats only);
-                             -1 = auto (3 for FP64, else 0; FP64 builds
-                             only 2 and 3)                                */
+                             -1 = auto (the fewest cells covered; ties:
+                             4 for 2-byte formats, 0 for FP32, 3 for
+                             FP64; FP64 builds only 2 and 3)              */
     int two_step_zseg;    /* z-planes per workgroup sweep; 0 = auto       */
     int two_step_xcds;    /* tile order of the two-step pass over the
                              XCDs, as xcd_groups; 0 = auto (8: each XCD
@@ -110,6 +111,10 @@ typedef struct imex_config {
     int two_step_prefetch; /* load the next plane's populations into
                              registers ahead of the barrier; 1 = on, 0 =
                              off, -1 = auto (on for 8- and 4-byte formats) */
+    int two_step_io;      /* periodic boxes: the pass's loads and stores
+                             through 1 = global addresses, 0 = buffer
+                             descriptors (direction offset in a scalar
+                             operand); -1 = auto (1)                     */
     int halo_direct;      /* IMEX_TILED: 1 = each tile's boundary slabs are
                              copied straight into its neighbours' ghosts
                              (one launch), 0 = packed into an exchange
@@ -135,7 +140,7 @@ typedef struct imex_info {
     int io16;
     int xcd_run;
     int two_step;             /* 1 if pairs of steps run as one pass      */
-    int two_step_tile, two_step_zseg, two_step_xcds, two_step_prefetch;
+    int two_step_tile, two_step_zseg, two_step_xcds, two_step_prefetch, two_step_io;
     int halo_direct;          /* IMEX_TILED: 1 direct ghost fill, 0 pack/unpack */
     int halo_depth;           /* IMEX_TILED: ghost layers filled (2 when two
                                  steps run per pass)                          */

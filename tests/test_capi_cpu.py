@@ -47,7 +47,7 @@ def test_strerror_and_defaults():
     assert c.halo_depth == 1 and c.dir_pad == -1 and c.nontemporal == -1 and c.xcd_groups == 0
     assert c.xcd_run == -1 and c.io16 == -1
     assert c.two_step == -1 and c.two_step_tile == -1 and c.two_step_zseg == 0 and c.two_step_xcds == 0
-    assert c.two_step_prefetch == -1 and c.halo_direct == -1
+    assert c.two_step_prefetch == -1 and c.halo_direct == -1 and c.two_step_io == -1
 
 
 @pytest.mark.skipif(gpu_available(), reason="checks the no-GPU error path")

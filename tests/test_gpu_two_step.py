@@ -55,7 +55,9 @@ def test_two_step_bitwise_equals_single_steps(prec, tile, pf, shape, zseg, steps
     if shape[0] % tx or shape[1] % ty:
         pytest.skip("tile does not divide the domain")
     d = _dom(*shape)
-    fa, fb, ca, cb = _pair(d, prec, steps, two_step_tile=tile, two_step_zseg=zseg, two_step_prefetch=pf)
+    io = (tile + pf + shape[2]) % 2          # both access paths across the cases
+    fa, fb, ca, cb = _pair(d, prec, steps, two_step_tile=tile, two_step_zseg=zseg, two_step_prefetch=pf,
+                           two_step_io=io)
     assert np.array_equal(fa, fb)
     assert np.array_equal(np.array(ca), np.array(cb))
 
@@ -240,6 +242,8 @@ def test_two_step_ragged_column_tiles(prec, tile, shape):
     if shape[0] < tx or shape[1] < ty:
         pytest.skip("box narrower than the tile")
     d = _dom(*shape)
-    fa, fb, ca, cb = _pair(d, prec, 7, two_step_tile=tile, two_step_zseg=3)
+    for io in (0, 1):
+        fa, fb, ca, cb = _pair(d, prec, 7, two_step_tile=tile, two_step_zseg=3, two_step_io=io)
+        assert np.array_equal(fa, fb)
     assert np.array_equal(fa, fb)
     assert np.array_equal(np.array(ca), np.array(cb))
