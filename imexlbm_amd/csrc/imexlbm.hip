@@ -8,6 +8,11 @@
 // writes the populations exactly once.  Anything that observes S (the
 // conserved sums, a download) gathers it through the pull offsets, which is
 // bit-identical to materialising it because streaming is a copy.
+//
+// Where the boxes allow it, pairs of fused steps run as one pass (k_step2:
+// the intermediate step lives in LDS), bit-identical to two launches.
+// imex_step hands the pairs out directly; Streaming() leaves at most one
+// fused step owed until the next call that needs the state.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
