@@ -230,10 +230,14 @@ __device__ inline void collide32(float (&g)[Q], float omega)
     g[0] = __builtin_fmaf(omega, (float)W_REST * A - g[0], g[0]);
     const float s[13] = {u, v, w, u + v, u - v, u + w, u - w, v + w, v - w,
                          (u + v) + w, (u - v) + w, (u + v) - w, (u - v) - w};
+    // w*A once per weight group; the symmetric part of the pair is then
+    // w*A + (4.5 w s) s, one multiply and one fused multiply-add
+    const float wA[3] = {(float)W_FACE * A, (float)W_EDGE * A, (float)W_CORNER * A};
 #pragma unroll
     for (int pr = 0; pr < 13; ++pr) {
         const int a = 1 + 2 * pr, b = a + 1;
-        const float B = __builtin_fmaf(4.5f * s[pr], s[pr], A) * (float)weight(a);
+        const int grp = a <= 6 ? 0 : (a <= 18 ? 1 : 2);
+        const float B = __builtin_fmaf(s[pr] * (float)(4.5 * weight(a)), s[pr], wA[grp]);
         const float C = s[pr] * weight3f(a);
         g[a] = __builtin_fmaf(omega, (B + C) - g[a], g[a]);
         g[b] = __builtin_fmaf(omega, (B - C) - g[b], g[b]);
