@@ -341,11 +341,31 @@ unsigned channel_mask(const int ch[3])
     return m;
 }
 
+// Directions of a ghost cell that a two-step pass reads, for the cell at
+// ghost layer kk[a] (1 or 2; 0: both, a slab two layers deep) on each axis a
+// the channel ch crosses: the pass pulls from layer 1 and the ring one cell
+// outside the box, so e (oriented along ch) must satisfy e_a >= kk[a] - 1:
+// the crossing and the in-layer directions at layer 1, the crossing ones only
+// at layer 2 (a subset, so a two-layer slab carries layer 1's set).
+unsigned layer_mask(const int ch[3], const int kk[3])
+{
+    unsigned m = 0;
+    for (int ii = 0; ii < Q; ++ii) {
+        const int e[3] = {EX[ii], EY[ii], EZ[ii]};
+        bool ok = true;
+        for (int a = 0; a < 3; ++a)
+            if (ch[a] != 0 && e[a] * ch[a] < std::max(kk[a], 1) - 1) ok = false;
+        if (ok) m |= 1u << ii;
+    }
+    return m;
+}
+
 // Halo plan for the tiled mode: for every tile and each of its 26 neighbour
 // channels (src/mpi_view_transfer.cpp:260-449), one pack segment (boundary
 // slab -> exchange buffer) and one unpack segment (exchange buffer -> the
-// neighbour's ghost slab on the opposite side).  Only the directions that
-// cross the channel are carried (9 per face, 3 per edge, 1 per corner).
+// neighbour's ghost slab on the opposite side).  One step per pass carries
+// the directions that cross the channel (9 per face, 3 per edge, 1 per
+// corner); two steps per pass fill two layers, each with what it needs.
 void build_halo(imex_ctx *c)
 {
     c->segs.clear();
@@ -373,42 +393,58 @@ void build_halo(imex_ctx *c)
                         const BoxDesc &t = c->boxes[I.box_begin + me];
                         const int n[3] = {b.nx, b.ny, b.nz};
                         const int nt[3] = {t.nx, t.ny, t.nz};
-                        int s0[3], d0[3], ext[3];
-                        for (int a = 0; a < 3; ++a) {
-                            if (ch[a] == 0) { s0[a] = 0; d0[a] = 0; ext[a] = n[a]; }
-                            else if (ch[a] > 0) { s0[a] = n[a] - d; d0[a] = -d; ext[a] = d; }
-                            else { s0[a] = 0; d0[a] = nt[a]; ext[a] = d; }
+                        // one step per pass: every filled layer at once, with the
+                        // crossing directions.  Two steps per pass: layers 1 and 2
+                        // of the y and z ghost axes apart, each with the directions
+                        // read there; along x both layers in one slab (they share
+                        // cache lines, as x is the contiguous axis)
+                        int ga[3], ng = 0;
+                        for (int a = c->halo_all ? 1 : 0; a < 3; ++a)
+                            if (ch[a]) ga[ng++] = a;
+                        const int combos = c->halo_all ? (1 << ng) : 1;
+                        for (int cb = 0; cb < combos; ++cb) {
+                            int kk[3] = {0, 0, 0};   // ghost layer per axis (0: layers 1..d)
+                            for (int j = 0; j < ng; ++j) kk[ga[j]] = c->halo_all ? 1 + ((cb >> j) & 1) : 0;
+                            // (x, or every axis for one step per pass, stays 0: layers 1..d)
+                            int s0[3], d0[3], ext[3];
+                            for (int a = 0; a < 3; ++a) {
+                                if (ch[a] == 0) { s0[a] = 0; d0[a] = 0; ext[a] = n[a]; }
+                                else if (kk[a] == 0 && ch[a] > 0) { s0[a] = n[a] - d; d0[a] = -d; ext[a] = d; }
+                                else if (kk[a] == 0) { s0[a] = 0; d0[a] = nt[a]; ext[a] = d; }
+                                else if (ch[a] > 0) { s0[a] = n[a] - kk[a]; d0[a] = -kk[a]; ext[a] = 1; }
+                                else { s0[a] = kk[a] - 1; d0[a] = nt[a] + kk[a] - 1; ext[a] = 1; }
+                            }
+                            const unsigned mask = c->halo_all ? layer_mask(ch, kk) : channel_mask(ch);
+                            const int ndir = __builtin_popcount(mask);
+                            HaloSeg ps{};
+                            ps.ex = ext[0]; ps.ey = ext[1]; ps.ez = ext[2];
+                            ps.ncell = ext[0] * ext[1] * ext[2];
+                            ps.mask = mask;
+                            ps.pack = 1;
+                            ps.s_off = b.org + s0[0] + (long long)s0[1] * b.sy + (long long)s0[2] * b.sz;
+                            ps.s_ds = b.dstride;
+                            ps.s_sy = b.sy; ps.s_sz = b.sz;
+                            ps.d_off = xoff;
+                            ps.d_ds = ps.ncell;
+                            ps.d_sy = ext[0]; ps.d_sz = ext[0] * ext[1];
+                            HaloSeg us = ps;
+                            us.pack = 0;
+                            us.s_off = xoff;
+                            us.s_ds = ps.ncell;
+                            us.s_sy = ext[0]; us.s_sz = ext[0] * ext[1];
+                            us.d_off = t.org + d0[0] + (long long)d0[1] * t.sy + (long long)d0[2] * t.sz;
+                            us.d_ds = t.dstride;
+                            us.d_sy = t.sy; us.d_sz = t.sz;
+                            xoff += round_up((long long)ndir * ps.ncell, 64);
+                            HaloSeg ds = ps;   // boundary slab of b -> ghost slab of t
+                            ds.pack = 2;
+                            ds.d_off = us.d_off;
+                            ds.d_ds = us.d_ds;
+                            ds.d_sy = us.d_sy; ds.d_sz = us.d_sz;
+                            c->segs.push_back(ps);
+                            unpack.push_back(us);
+                            direct.push_back(ds);
                         }
-                        const unsigned mask = c->halo_all ? (1u << Q) - 1 : channel_mask(ch);
-                        const int ndir = __builtin_popcount(mask);
-                        HaloSeg ps{};
-                        ps.ex = ext[0]; ps.ey = ext[1]; ps.ez = ext[2];
-                        ps.ncell = ext[0] * ext[1] * ext[2];
-                        ps.mask = mask;
-                        ps.pack = 1;
-                        ps.s_off = b.org + s0[0] + (long long)s0[1] * b.sy + (long long)s0[2] * b.sz;
-                        ps.s_ds = b.dstride;
-                        ps.s_sy = b.sy; ps.s_sz = b.sz;
-                        ps.d_off = xoff;
-                        ps.d_ds = ps.ncell;
-                        ps.d_sy = ext[0]; ps.d_sz = ext[0] * ext[1];
-                        HaloSeg us = ps;
-                        us.pack = 0;
-                        us.s_off = xoff;
-                        us.s_ds = ps.ncell;
-                        us.s_sy = ext[0]; us.s_sz = ext[0] * ext[1];
-                        us.d_off = t.org + d0[0] + (long long)d0[1] * t.sy + (long long)d0[2] * t.sz;
-                        us.d_ds = t.dstride;
-                        us.d_sy = t.sy; us.d_sz = t.sz;
-                        xoff += round_up((long long)ndir * ps.ncell, 64);
-                        HaloSeg ds = ps;   // boundary slab of b -> ghost slab of t
-                        ds.pack = 2;
-                        ds.d_off = us.d_off;
-                        ds.d_ds = us.d_ds;
-                        ds.d_sy = us.d_sy; ds.d_sz = us.d_sz;
-                        c->segs.push_back(ps);
-                        unpack.push_back(us);
-                        direct.push_back(ds);
                     }
         }
     }
@@ -1079,7 +1115,7 @@ int imex_create(imex_ctx **out, const imex_config *cfg_in, const imex_domain *do
             // the second step pulls through the first step's ring: two ghost
             // layers, with every direction of the first one
             c->halo_all = true;
-            c->halo_depth = std::max(c->halo_depth, 2);
+            c->halo_depth = 2;
         }
         c->ts_zseg = cfg.two_step_zseg > 0 ? cfg.two_step_zseg : (zdef > 0 ? zdef : 16);
         c->ts_xcds = cfg.two_step_xcds > 0 ? cfg.two_step_xcds : 8;
