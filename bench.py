@@ -148,6 +148,10 @@ def roofline(h, traffic, copy_gbs):
     if traffic:
         out["hbm_gbs"] = round(traffic / launch_s / 1e9, 1)
         out["hbm_frac"] = round(traffic / launch_s / 1e9 / HBM_PEAK_GBS, 4)
+    if ups == 2:
+        out["note"] = ("two lattice updates per launch (temporal blocking): achieved counts "
+                       "2*27*sizeof bytes per cell per update, HBM moves traffic bytes per launch "
+                       "(hbm_gbs)")
     return out
 
 
