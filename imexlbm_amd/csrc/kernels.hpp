@@ -465,7 +465,8 @@ struct PassBox { int wg0, box, tiles_x, ntile; };
 // run past nx or ny (TX <= nx, TY <= ny): its ring wraps onto real cells and
 // its output cells beyond the box are not stored.  Otherwise the tile divides
 // the box, and the ring and the planes z0 - 1 and z1 are ghost cells, filled
-// two layers deep with all 27 directions before the pass.
+// two layers deep (with the directions each layer is read for) before the
+// pass.
 template <class S, bool WRAP, int TX, int TY, int NTH, bool NT, bool PF, bool GIO = true>
 __global__ void __launch_bounds__(NTH)
 k_step2(const S *__restrict__ src, S *__restrict__ dst, const BoxDesc *__restrict__ boxes,

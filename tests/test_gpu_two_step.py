@@ -247,3 +247,24 @@ def test_two_step_ragged_column_tiles(prec, tile, shape):
         assert np.array_equal(fa, fb)
     assert np.array_equal(fa, fb)
     assert np.array_equal(np.array(ca), np.array(cb))
+
+
+def test_two_step_fp64_128_matches_oracle_bitwise():
+    # the default FP64 pass (32x8 columns, 16-plane sweeps) at 128^3 against
+    # the CPU restatement of the reference, populations bit for bit
+    from oracle import oracle
+    d = _dom(128, re=1600)
+    with _ctx(d, precision="fp64") as c:
+        c.initialize()
+        assert c.info()["two_step"] == 1
+        c.step(10)
+        got = c.get_f()
+        cons = c.conserved()
+    o = oracle.Oracle(d["nx"], d["ny"], d["nz"], d["tau0"], d["rho0"], d["u0"])
+    o.initialize()
+    o.step(10)
+    want = o.get_f()
+    wc = o.conserved_compensated()
+    o.close()
+    assert np.array_equal(got, want)
+    assert abs(cons[4] - wc[4]) / wc[4] < 1e-12
