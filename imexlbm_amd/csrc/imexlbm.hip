@@ -1217,7 +1217,7 @@ int imex_streaming(imex_ctx *c)
         c->pending = false;
         if (c->state_S) {
             TRY(do_collide(c));
-        } else if (c->two_step && !c->profile) {
+        } else if (c->two_step) {
             // fused steps go out in pairs, one pass each; an odd one left
             // over is launched by the next call that needs the state
             if (++c->deferred == 2) {

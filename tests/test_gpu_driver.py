@@ -152,3 +152,15 @@ def test_driver_two_step_pairs_match_single_steps(tmp_path, prec):
     off, _ = _run(tmp_path, deck, IMEXLBM_PRECISION=prec, IMEXLBM_TWO_STEP=0)
     assert len(on) == len(off) == 8
     assert on == off
+
+
+def test_driver_phase_profile_two_step(tmp_path):
+    # the profiler times the two-step passes the driver's Streaming() pairs
+    # launch: reports every 5 steps, 39 timed steps after step 1
+    deck = "1.0  0.1  1600\n0.05  40  5\n64  64  32\n"
+    got, out = _run(tmp_path, deck, IMEXLBM_PROFILE=1, IMEXLBM_TWO_STEP=1)
+    ref, _ = _run(tmp_path, deck, IMEXLBM_TWO_STEP=0)
+    assert "=== per-phase profile: 1 GPU, 39 timed steps ===" in out
+    rows = {l[:34].strip(): l[34:].split() for l in out.splitlines() if l.startswith("two fused steps")}
+    assert int(rows["two fused steps in one pass"][0]) >= 15
+    assert got == ref
