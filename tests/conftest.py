@@ -1,3 +1,4 @@
+import json
 import os
 import sys
 
@@ -26,3 +27,18 @@ def oracle_mod():
     from oracle import oracle
     oracle.build()
     return oracle
+
+
+def golden(name: str) -> dict:
+    """A fixture under tests/golden (data the reference's own tests hold)."""
+    with open(os.path.join(ROOT, "tests", "golden", name)) as fh:
+        return json.load(fh)
+
+
+REGRESSION = golden("regression_deck.json")
+
+
+def ke_matches_reference(ke_values) -> bool:
+    """The first three reported KE values, to the digits the reference quotes."""
+    want = REGRESSION["ke_first_reports"]
+    return [float(f"{k:.4g}") for k in ke_values[:3]] == [float(f"{w:.4g}") for w in want]

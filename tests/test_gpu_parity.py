@@ -8,7 +8,7 @@ must agree to 1e-12 relative (mass, KE) and 1e-12 absolute (momentum).
 import numpy as np
 import pytest
 
-from tests.conftest import ROOT  # noqa: F401
+from tests.conftest import ROOT, ke_matches_reference  # noqa: F401
 
 pytestmark = pytest.mark.gpu
 
@@ -83,7 +83,7 @@ def test_regression_deck_through_the_abi():
             c.streaming()
             cons.append(c.conserved())
     cons = np.array(cons)
-    assert [float(f"{k:.4g}") for k in cons[:3, 4]] == [9.852, 10.22, 9.889]
+    assert ke_matches_reference(cons[:, 4])
     m0 = cons[0, 0]
     assert np.abs((cons[:, 0] - m0) / m0).max() < 1e-12
     assert np.abs(cons[:, 1:4]).max() < 1e-10
