@@ -16,11 +16,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -373,6 +375,8 @@ void build_halo(imex_ctx *c)
     c->unpack_blocks.clear();
     c->direct_blocks.clear();
     std::vector<HaloSeg> direct;
+    std::vector<std::array<int, 5>> direct_key;   // box, channel (x, y, z), layer combination
+    std::map<std::array<int, 5>, int> direct_at;
     const int d = c->halo_depth;
     long long xoff = 0;
     std::vector<HaloSeg> unpack;
@@ -421,6 +425,7 @@ void build_halo(imex_ctx *c)
                             ps.ncell = ext[0] * ext[1] * ext[2];
                             ps.mask = mask;
                             ps.pack = 1;
+                            ps.partner = -1;
                             ps.s_off = b.org + s0[0] + (long long)s0[1] * b.sy + (long long)s0[2] * b.sz;
                             ps.s_ds = b.dstride;
                             ps.s_sy = b.sy; ps.s_sz = b.sz;
@@ -443,6 +448,8 @@ void build_halo(imex_ctx *c)
                             ds.d_sy = us.d_sy; ds.d_sz = us.d_sz;
                             c->segs.push_back(ps);
                             unpack.push_back(us);
+                            direct_at[{bi, cx, cy, cz, cb}] = (int)direct.size();
+                            direct_key.push_back({I.box_begin + me, -cx, -cy, -cz, cb});
                             direct.push_back(ds);
                         }
                     }
@@ -455,8 +462,17 @@ void build_halo(imex_ctx *c)
         for (int c0 = 0; c0 < unpack[s].ncell; c0 += 256) c->unpack_blocks.push_back({npack + s, c0});
     c->segs.insert(c->segs.end(), unpack.begin(), unpack.end());
     const int nsu = (int)c->segs.size();
-    for (int s = 0; s < (int)direct.size(); ++s)
+    // pair each direct segment with its reverse: (b -> t across ch) with
+    // (t -> b across -ch), which has the same layer combination and extents
+    for (int s = 0; s < (int)direct.size(); ++s) {
+        const auto it = direct_at.find(direct_key[s]);
+        const int q = it == direct_at.end() ? -1 : it->second;
+        const bool same = q >= 0 && direct[q].ex == direct[s].ex && direct[q].ey == direct[s].ey &&
+                          direct[q].ez == direct[s].ez;
+        if (same && q > s) direct[s].partner = nsu + q;
+        if (same && q < s) continue;               // copied with its partner
         for (int c0 = 0; c0 < direct[s].ncell; c0 += 256) c->direct_blocks.push_back({nsu + s, c0});
+    }
     c->segs.insert(c->segs.end(), direct.begin(), direct.end());
     c->xbuf_elems = xoff;
     c->halo_bytes = 0;

@@ -55,6 +55,8 @@ struct HaloSeg {
     unsigned mask;              // directions carried (bit ii)
     int pack;                   // 1: tile -> buffer, 0: buffer -> tile,
                                 // 2: tile boundary -> neighbour's ghosts
+    int partner;                // direct fill: the reverse segment, copied by
+                                // the same thread for the same cell (-1: none)
 };
 
 enum { OP_COLLIDE = 0, OP_FUSED = 1, OP_STREAM = 2 };
@@ -811,34 +813,99 @@ k_import(S *__restrict__ dst, const BoxDesc *__restrict__ boxes, int box,
 // direct fill (pack == 2: a tile's boundary slab straight into the
 // neighbour's ghost slab of the same pool; the slabs read are interior cells,
 // those written ghost cells, so no cell is both).  Each workgroup copies up
-// to blockDim cells of one segment for all of its directions.  Copies are
-// bit-exact moves of the stored format.
+// to blockDim cells of one segment for all of its directions, loading every
+// direction before storing any.  Copies are bit-exact moves of the stored
+// format.
+//
+// A direct segment (tile b -> the ghosts of its neighbour t across channel
+// ch) and its partner (t -> the ghosts of b across -ch) have the same extents
+// and read and write the same cache lines: b's boundary cells and b's ghost
+// cells on that side lie in the same rows, and so do t's.  Along x, the
+// contiguous axis, a row holds only a few bytes of either, so one thread
+// copies the same cell of both segments: every line is read and then written
+// while it is still in the cache, instead of being fetched once per segment
+// and written back partially.
 // ---------------------------------------------------------------------------
+struct HaloCell {
+    long long so, dofs;
+};
+
+__device__ inline HaloCell halo_cell(const HaloSeg &sg, int c)
+{
+    const int x = c % sg.ex;
+    const int y = (c / sg.ex) % sg.ey;
+    const int z = c / (sg.ex * sg.ey);
+    return {sg.s_off + x + (long long)y * sg.s_sy + (long long)z * sg.s_sz,
+            sg.d_off + x + (long long)y * sg.d_sy + (long long)z * sg.d_sz};
+}
+
+// s_waitcnt operand (gfx9 encoding) that waits for every vector memory
+// access: vmcnt 0, expcnt and lgkmcnt at their maxima (not waited for).
+constexpr int VMCNT0 = (7 << 4) | (15 << 8);
+
+// The bits of one stored value.  A 2-byte value is held in a 32-bit register
+// of its own: packing two into one would make each load wait for the last.
+template <class S> struct HaloWord { using type = S; };
+template <> struct HaloWord<_Float16> { using type = unsigned; };
+template <> struct HaloWord<bf16_t> { using type = unsigned; };
+
+template <class S, class W = typename HaloWord<S>::type>
+__device__ inline void halo_load(const S *__restrict__ src, const HaloSeg &sg, const HaloCell &hc, W (&v)[Q])
+{
+    int slot = 0;
+#pragma unroll
+    for (int ii = 0; ii < Q; ++ii)
+        if (sg.mask & (1u << ii)) {
+            const S *p = src + hc.so + (long long)(sg.pack ? ii : slot) * sg.s_ds;
+            if constexpr (sizeof(S) == 2) v[ii] = *reinterpret_cast<const unsigned short *>(p);
+            else v[ii] = *p;
+            ++slot;
+        }
+}
+
+template <class S, class W = typename HaloWord<S>::type>
+__device__ inline void halo_store(S *__restrict__ dst, const HaloSeg &sg, const HaloCell &hc, const W (&v)[Q])
+{
+    int slot = 0;
+#pragma unroll
+    for (int ii = 0; ii < Q; ++ii)
+        if (sg.mask & (1u << ii)) {
+            S *p = dst + hc.dofs + (long long)(sg.pack == 1 ? slot : ii) * sg.d_ds;
+            if constexpr (sizeof(S) == 2) *reinterpret_cast<unsigned short *>(p) = (unsigned short)v[ii];
+            else *p = v[ii];
+            ++slot;
+        }
+}
+
 template <class S>
 __global__ void __launch_bounds__(256)
 k_halo(const S *__restrict__ src, S *__restrict__ dst, const HaloSeg *__restrict__ segs,
        const int2 *__restrict__ blocks, int nblocks)
 {
-    for (int b = blockIdx.x; b < nblocks; b += gridDim.x) {
-        const int2 bk = blocks[b];
-        const HaloSeg &sg = segs[bk.x];
-        const int c = bk.y + (int)threadIdx.x;
-        if (c >= sg.ncell) continue;
-        const int x = c % sg.ex;
-        const int y = (c / sg.ex) % sg.ey;
-        const int z = c / (sg.ex * sg.ey);
-        const long long so = sg.s_off + x + (long long)y * sg.s_sy + (long long)z * sg.s_sz;
-        const long long dofs = sg.d_off + x + (long long)y * sg.d_sy + (long long)z * sg.d_sz;
-        int slot = 0;
-#pragma unroll
-        for (int ii = 0; ii < Q; ++ii) {
-            if (sg.mask & (1u << ii)) {
-                const long long sdir = sg.pack ? ii : slot;
-                const long long ddir = sg.pack == 1 ? slot : ii;
-                dst[dofs + ddir * sg.d_ds] = src[so + sdir * sg.s_ds];
-                ++slot;
-            }
-        }
+    using W = typename HaloWord<S>::type;
+    // one workgroup per block entry (the grid is nblocks)
+    if ((int)blockIdx.x >= nblocks) return;
+    const int2 bk = blocks[blockIdx.x];
+    const HaloSeg sg = segs[bk.x];
+    const int c = bk.y + (int)threadIdx.x;
+    if (c >= sg.ncell) return;
+    const HaloCell hc = halo_cell(sg, c);
+    W v[Q];
+    halo_load(src, sg, hc, v);
+    if (sg.partner >= 0) {
+        const HaloSeg sp = segs[sg.partner];
+        const HaloCell hp = halo_cell(sp, c);
+        W w[Q];
+        halo_load(src, sp, hp, w);
+        // every load has landed: the stores then need no wait of their own
+        // (the compiler cannot count the loads taken across the per-direction
+        // branches and would wait for all outstanding accesses before each)
+        __builtin_amdgcn_s_waitcnt(VMCNT0);
+        halo_store(dst, sg, hc, v);
+        halo_store(dst, sp, hp, w);
+    } else {
+        __builtin_amdgcn_s_waitcnt(VMCNT0);
+        halo_store(dst, sg, hc, v);
     }
 }
 
