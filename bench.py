@@ -47,11 +47,11 @@ def tau0(u0, n, re):
 
 
 def run_mode(capi, prec, n, re, u0, steps, warmup, autotune, use_graph, mode="wrap", tiles=8, halo=1,
-             two_step=-1):
+             two_step=-1, whole_x=0):
     dom = {"nx": n, "ny": n, "nz": n, "tau0": tau0(u0, n, re), "rho0": 1.0, "u0": u0}
     kw = {"two_step": two_step}
     if mode == "tiled":
-        kw.update({"mode": capi.TILED, "ntiles": tiles, "halo_depth": halo})
+        kw.update({"mode": capi.TILED, "ntiles": tiles, "halo_depth": halo, "whole_x": whole_x})
     with capi.Context(dom, precision=prec, use_graph=int(use_graph), **kw) as ctx:
         ctx.initialize()
         ctx.step(1)                       # first collision: the state becomes post-collision
@@ -173,6 +173,8 @@ def main():
                     help="periodic wrap in the kernel, or subdomain tiles with device pack/unpack")
     ap.add_argument("--tiles", type=int, default=8)
     ap.add_argument("--halo", type=int, default=1, help="ghost layers filled per exchange (tiled)")
+    ap.add_argument("--whole-x", action="store_true",
+                    help="tiled: cut y and z only and wrap x in the kernels (no x ghosts)")
     ap.add_argument("--two-step", choices=("auto", "on", "off"), default="auto",
                     help="two steps per pass (temporal blocking): auto lets the autotuner decide")
     args = ap.parse_args()
@@ -195,7 +197,7 @@ def main():
         print(f"bench: {m} {args.n}^3 steps={args.steps} warmup={args.warmup}", file=sys.stderr, flush=True)
         results[m] = run_mode(capi, m, args.n, args.re, args.u0, args.steps, args.warmup,
                               not args.no_autotune, args.graph, args.mode, args.tiles, args.halo,
-                              {"auto": -1, "on": 1, "off": 0}[args.two_step])
+                              {"auto": -1, "on": 1, "off": 0}[args.two_step], int(args.whole_x))
         print(f"bench: {m} {results[m]['mlups']} MLUPS, {results[m]['achieved_gbs']} GB/s",
               file=sys.stderr, flush=True)
 
@@ -223,7 +225,8 @@ def main():
             "workload": f"D3Q27 BGK Taylor-Green vortex {args.n}^3 periodic, Re={args.re}, "
                         f"u0={args.u0}, rho0=1, fused pull-stream+collide, "
                         + ("in-kernel wrap" if args.mode == "wrap" else
-                           f"{args.tiles} tiles, {args.halo}-layer device pack/unpack"),
+                           f"{args.tiles} tiles{' (x whole)' if args.whole_x else ''}, "
+                           f"{args.halo}-layer device pack/unpack"),
             "storage": args.headline,
             "cells": args.n ** 3,
             "parallelism": "single GPU",

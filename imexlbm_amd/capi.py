@@ -39,7 +39,7 @@ class Config(ctypes.Structure):
         "precision", "mode", "ntiles", "halo_depth", "block", "cells_per_thread",
         "persistent", "nontemporal", "dir_pad", "use_graph", "sequential", "device",
         "xcd_groups", "contiguous", "io16", "profile", "xcd_run", "two_step", "two_step_tile",
-        "two_step_zseg", "two_step_xcds", "two_step_prefetch", "two_step_io", "halo_direct")]
+        "two_step_zseg", "two_step_xcds", "two_step_prefetch", "two_step_io", "halo_direct", "whole_x")]
 
 
 class Info(ctypes.Structure):
@@ -49,7 +49,7 @@ class Info(ctypes.Structure):
                     "cells_per_thread", "persistent", "nontemporal", "dir_pad", "use_graph",
                     "grid", "rx", "ry", "rz", "xcd_groups", "contiguous", "io16", "xcd_run",
                     "two_step", "two_step_tile", "two_step_zseg", "two_step_xcds", "two_step_prefetch",
-                    "two_step_io", "halo_direct", "halo_depth")]
+                    "two_step_io", "halo_direct", "halo_depth", "whole_x")]
 
 
 class ImexError(RuntimeError):

@@ -79,11 +79,12 @@ struct Instance {
 
 // LBM::setup_Cartesian's factorisation (src/mpi_view_transfer.cpp:55-91): the
 // first strict minimum of the spread of the int local extents, x outer.
-void cartesian(int n, const int gl[3], int r[3])
+// whole_x keeps x uncut (rx = 1) and factors y and z by the same rule.
+void cartesian(int n, const int gl[3], int r[3], bool whole_x = false)
 {
     double min_e = 1e16;
     r[0] = r[1] = r[2] = 1;
-    for (int i = 1; i <= n; ++i) {
+    for (int i = 1; i <= (whole_x ? 1 : n); ++i) {
         if (n % i) continue;
         for (int j = 1; j <= n / i; ++j) {
             if ((n / i) % j) continue;
@@ -121,6 +122,7 @@ struct imex_ctx {
     int prec = IMEX_F64;
     int ssize = 8;
     bool tiled = false;
+    bool whole_x = false;        // tiled: tiles keep x whole, kernels wrap x
     int device = 0;
     int ncu = 256;
     hipStream_t stream = nullptr;
@@ -234,7 +236,7 @@ int layout_boxes(imex_ctx *c)
         const int gl[3] = {I.dom.nx, I.dom.ny, I.dom.nz};
         int r[3] = {1, 1, 1};
         if (c->tiled) {
-            cartesian(c->cfg.ntiles, gl, r);
+            cartesian(c->cfg.ntiles, gl, r, c->whole_x);
             std::copy(r, r + 3, c->rgrid);
         }
         I.box_begin = (int)c->boxes.size();
@@ -256,7 +258,14 @@ int layout_boxes(imex_ctx *c)
             b.nx = n[0]; b.ny = n[1]; b.nz = n[2];
             // f[z][ii][y][x]: 27 direction sub-planes per z-plane
             long long rows, zplanes, org_xy, zorg;
-            if (c->tiled) {
+            if (c->tiled && c->whole_x) {
+                // ghost rows and planes only: x wraps in the kernels
+                b.sy = (int)round_up(n[0], align);
+                rows = n[1] + 6;
+                zplanes = n[2] + 6;
+                org_xy = 3LL * b.sy;
+                b.wrap_x = 1;
+            } else if (c->tiled) {
                 const long long xoff = std::max<long long>(align, 3);
                 b.sy = (int)round_up(xoff + n[0] + 3, align);
                 rows = n[1] + 6;
@@ -389,6 +398,7 @@ void build_halo(imex_ctx *c)
                 for (int cy = -1; cy <= 1; ++cy)
                     for (int cz = -1; cz <= 1; ++cz) {
                         if (!cx && !cy && !cz) continue;
+                        if (cx && c->whole_x) continue;   // x wraps in the kernels
                         const int ch[3] = {cx, cy, cz};
                         int q[3];
                         for (int a = 0; a < 3; ++a) q[a] = (p[a] + ch[a] + c->rgrid[a]) % c->rgrid[a];
@@ -1052,6 +1062,7 @@ int imex_create(imex_ctx **out, const imex_config *cfg_in, const imex_domain *do
     if (cfg.ntiles <= 0) cfg.ntiles = 8;
     if (cfg.halo_depth <= 0) cfg.halo_depth = 1;
     if (cfg.halo_depth > 3) return fail(IMEX_EINVAL, "halo_depth must be 1..3");
+    if (cfg.whole_x != 0 && cfg.whole_x != 1) return fail(IMEX_EINVAL, "whole_x must be 0 or 1");
     if (cfg.block != 0 && (cfg.block % 64 != 0 || cfg.block > 512)) return fail(IMEX_EINVAL, "block must be a multiple of 64, at most 512");
     if (cfg.cells_per_thread < 0 || cfg.cells_per_thread > 16) return fail(IMEX_EINVAL, "cells_per_thread must be 0..16");
     for (int d = 0; d < ndomains; ++d) {
@@ -1070,6 +1081,7 @@ int imex_create(imex_ctx **out, const imex_config *cfg_in, const imex_domain *do
     c->prec = cfg.precision;
     c->ssize = storage_bytes(cfg.precision);
     c->tiled = cfg.mode == IMEX_TILED;
+    c->whole_x = c->tiled && cfg.whole_x != 0;
     c->halo_depth = cfg.halo_depth;
     c->halo_direct = cfg.halo_direct != 0;
     c->block = cfg.block ? cfg.block : 256;
@@ -1634,6 +1646,7 @@ int imex_query(imex_ctx *c, imex_info *info)
     info->two_step_prefetch = c->ts_pf;
     info->two_step_io = c->ts_buf ? 0 : 1;
     info->halo_direct = c->tiled ? (c->halo_direct ? 1 : 0) : 0;
+    info->whole_x = c->whole_x ? 1 : 0;
     info->halo_depth = c->tiled ? c->halo_depth : 0;
     info->grid = c->tiled ? 1 : 0;
     info->rx = c->rgrid[0]; info->ry = c->rgrid[1]; info->rz = c->rgrid[2];

@@ -39,6 +39,7 @@ struct BoxDesc {
     int org_xy;         // in-plane offset of interior cell (0,0) from the row-0 start
     long long base;     // element offset of the box allocation (ghosts included)
     long long zorg;     // offset of interior z-plane 0 from base (3*sz with ghosts)
+    int wrap_x;         // box with ghosts in y and z only: x wraps in the kernels
 };
 
 struct WorkTile { int box, z, p0, n; };
@@ -76,7 +77,8 @@ __device__ inline Nbr neighbours(const BoxDesc &d, int i, int j, int k)
         n.zm = (k == 0) ? (long long)(d.nz - 1) * d.sz : -(long long)d.sz;
         n.zp = (k == d.nz - 1) ? (long long)(1 - d.nz) * d.sz : (long long)d.sz;
     } else {
-        n.xm = -1; n.xp = 1;
+        n.xm = (d.wrap_x && i == 0) ? d.nx - 1 : -1;
+        n.xp = (d.wrap_x && i == d.nx - 1) ? 1 - d.nx : 1;
         n.ym = -d.sy; n.yp = d.sy;
         n.zm = -d.sz; n.zp = d.sz;
     }
@@ -468,7 +470,7 @@ struct PassBox { int wg0, box, tiles_x, ntile; };
 // its output cells beyond the box are not stored.  Otherwise the tile divides
 // the box, and the ring and the planes z0 - 1 and z1 are ghost cells, filled
 // two layers deep (with the directions each layer is read for) before the
-// pass.
+// pass; a box that keeps x whole (wrap_x) wraps the ring in x instead.
 template <class S, bool WRAP, int TX, int TY, int NTH, bool NT, bool PF, bool GIO = true>
 __global__ void __launch_bounds__(NTH)
 k_step2(const S *__restrict__ src, S *__restrict__ dst, const BoxDesc *__restrict__ boxes,
@@ -503,6 +505,8 @@ k_step2(const S *__restrict__ src, S *__restrict__ dst, const BoxDesc *__restric
             zp = z == d.nz - 1 ? 0 : z + 1;
             if (x < 0) x += d.nx; else if (x >= d.nx) x -= d.nx;
             if (y < 0) y += d.ny; else if (y >= d.ny) y -= d.ny;
+        } else if (d.wrap_x) {
+            if (x < 0) x += d.nx; else if (x >= d.nx) x -= d.nx;
         }
         const Planes<S, GIO> P(sbox + (long long)zm * d.sz, sbox + (long long)z * d.sz,
                                sbox + (long long)zp * d.sz, dbox);

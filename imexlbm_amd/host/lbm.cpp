@@ -53,6 +53,7 @@ LBM::LBM(lbm_comm comm_, int sx, int sy, int sz, double &tau, double &rho0_, dou
     cfg.halo_depth = env_int("IMEXLBM_HALO", 1);
     cfg.profile = env_int("IMEXLBM_PROFILE", 0);
     cfg.two_step = env_int("IMEXLBM_TWO_STEP", -1);
+    cfg.whole_x = env_int("IMEXLBM_WHOLE_X", 0);
 
     // one rank owns the whole domain (src/mpi_view_transfer.cpp:163-195 at rx=ry=rz=1)
     l_l[0] = glx; l_l[1] = gly; l_l[2] = glz;

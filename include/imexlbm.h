@@ -121,6 +121,11 @@ typedef struct imex_config {
                              buffer and unpacked from it (two launches:
                              the shape of a cross-GPU exchange); -1 = auto
                              (1)                                         */
+    int whole_x;          /* IMEX_TILED: 1 = tiles cut y and z only (the
+                             setup_Cartesian factorisation with rx = 1)
+                             and the kernels wrap x themselves, so no x
+                             ghosts are kept or filled; 0 = the reference's
+                             factorisation over x, y and z (default)     */
 } imex_config;
 
 typedef struct imex_info {
@@ -144,6 +149,8 @@ typedef struct imex_info {
     int halo_direct;          /* IMEX_TILED: 1 direct ghost fill, 0 pack/unpack */
     int halo_depth;           /* IMEX_TILED: ghost layers filled (2 when two
                                  steps run per pass)                          */
+    int whole_x;              /* IMEX_TILED: 1 if tiles keep x whole and the
+                                 kernels wrap it                              */
 } imex_info;
 
 /* Defaults for every field of imex_config: FP64, wrap, 256-thread

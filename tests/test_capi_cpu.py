@@ -48,6 +48,7 @@ def test_strerror_and_defaults():
     assert c.xcd_run == -1 and c.io16 == -1
     assert c.two_step == -1 and c.two_step_tile == -1 and c.two_step_zseg == 0 and c.two_step_xcds == 0
     assert c.two_step_prefetch == -1 and c.halo_direct == -1 and c.two_step_io == -1
+    assert c.whole_x == 0
 
 
 @pytest.mark.skipif(gpu_available(), reason="checks the no-GPU error path")

@@ -164,6 +164,39 @@ def test_tiled_matches_wrap_bitwise(ntiles, depth, n, direct):
     assert abs(cw[0] - ct[0]) / cw[0] <= 1e-13 and abs(cw[4] - ct[4]) / cw[4] <= 1e-13
 
 
+@pytest.mark.parametrize("direct", [1, 0])
+@pytest.mark.parametrize("ntiles,depth,n,two", [(8, 1, 16, 0), (8, 3, 16, 0), (4, 2, 17, 0), (12, 1, 18, 0),
+                                                (8, 3, 64, 1)])
+def test_tiled_whole_x_matches_wrap_bitwise(ntiles, depth, n, two, direct):
+    # tiles cut y and z only; x wraps inside the kernels, so no x ghosts are
+    # filled.  Populations equal the periodic single-box run bit for bit.
+    from imexlbm_amd import capi
+    d = _dom(n, re=400)
+    with _ctx(d, precision="fp64", two_step=0) as w:
+        w.initialize(); w.step(12)
+        fw, cw = w.get_f(), w.conserved()
+    with _ctx(d, precision="fp64", mode=capi.TILED, ntiles=ntiles, halo_depth=depth, halo_direct=direct,
+              whole_x=1, two_step=two) as t:
+        t.initialize(); t.step(12)
+        ft, ct = t.get_f(), t.conserved()
+        info = t.info()
+    assert info["ntiles"] == ntiles and info["whole_x"] == 1 and info["two_step"] == two
+    assert info["rx"] == 1 and info["ry"] * info["rz"] == ntiles
+    assert np.array_equal(fw, ft)
+    assert abs(cw[0] - ct[0]) / cw[0] <= 1e-13 and abs(cw[4] - ct[4]) / cw[4] <= 1e-13
+
+
+def test_tiled_whole_x_geometry():
+    # setup_Cartesian's rule over y and z with rx = 1: for a cube the first
+    # strict minimum of the spread puts the larger factor on z
+    from imexlbm_amd import capi
+    d = _dom(32)
+    for n, want in ((2, (1, 1, 2)), (4, (1, 2, 2)), (8, (1, 2, 4))):
+        with _ctx(d, mode=capi.TILED, ntiles=n, whole_x=1) as t:
+            i = t.info()
+            assert (i["rx"], i["ry"], i["rz"]) == want
+
+
 def test_tiled_geometry_follows_setup_cartesian():
     from imexlbm_amd import capi
     from oracle import oracle
@@ -278,10 +311,11 @@ def test_get_macro_matches_moments_of_oracle_state():
         assert np.abs(got - want).max() < 1e-15
 
 
-@pytest.mark.parametrize("direct,two", [(1, -1), (0, 0)])
-def test_tiled_256_three_layer_halo_matches_wrap(direct, two):
-    # the 256^3 deck's decomposition check: 2x2x2 tiles, 3-layer ghost fill
-    # (direct with two steps per pass, or pack/unpack one step per pass)
+@pytest.mark.parametrize("direct,two,whole_x", [(1, -1, 0), (0, 0, 0), (1, -1, 1)])
+def test_tiled_256_three_layer_halo_matches_wrap(direct, two, whole_x):
+    # the 256^3 deck's decomposition check: 2x2x2 tiles (1x2x4 keeping x
+    # whole), 3-layer ghost fill (direct with two steps per pass, or
+    # pack/unpack one step per pass)
     from imexlbm_amd import capi
     d = _dom(256, re=1600)
     with _ctx(d, precision="fp64") as w:
@@ -289,11 +323,12 @@ def test_tiled_256_three_layer_halo_matches_wrap(direct, two):
         cw = w.conserved()
         fw = w.get_f()
     with _ctx(d, precision="fp64", mode=capi.TILED, ntiles=8, halo_depth=3, halo_direct=direct,
-              two_step=two) as t:
+              two_step=two, whole_x=whole_x) as t:
         t.initialize(); t.step(6)
         ct = t.conserved()
         info = t.info()
         assert info["ntiles"] == 8 and info["two_step"] == (1 if two else 0)
+        assert (info["rx"], info["ry"], info["rz"]) == ((1, 2, 4) if whole_x else (2, 2, 2))
         assert np.array_equal(fw, t.get_f())
     del fw
     assert abs(cw[0] - ct[0]) / cw[0] <= 1e-13 and abs(cw[4] - ct[4]) / cw[4] <= 1e-13

@@ -162,18 +162,21 @@ def test_two_step_eligibility():
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32", "fp16", "bf16"])
-@pytest.mark.parametrize("ntiles,halo,direct", [(2, 1, 1), (4, 2, 0), (8, 3, 1), (8, 1, 0)])
-def test_tiled_two_step_equals_wrap_single_steps(prec, ntiles, halo, direct):
-    # tiles exchange two ghost layers with all 27 directions once per pass;
-    # populations must equal the periodic single-box run bit for bit
+@pytest.mark.parametrize("ntiles,halo,direct,whole_x", [(2, 1, 1, 0), (4, 2, 0, 0), (8, 3, 1, 0), (8, 1, 0, 0),
+                                                        (8, 3, 1, 1), (4, 2, 0, 1)])
+def test_tiled_two_step_equals_wrap_single_steps(prec, ntiles, halo, direct, whole_x):
+    # tiles exchange two ghost layers once per pass (whole_x: in y and z
+    # only, x wraps in the kernel); populations must equal the periodic
+    # single-box run bit for bit
     from imexlbm_amd import capi
     d = _dom(64, ny=64, nz=24)
     with _ctx(d, precision=prec, mode=capi.TILED, ntiles=ntiles, halo_depth=halo, two_step=1,
-              halo_direct=direct) as a:
+              halo_direct=direct, whole_x=whole_x) as a:
         a.initialize()
         a.step(7)
         info = a.info()
         assert info["two_step"] == 1 and info["halo_depth"] == 2 and info["halo_direct"] == direct
+        assert info["whole_x"] == whole_x and (info["rx"] == 1 or not whole_x)
         fa, ca = a.get_f(), a.conserved()
         a.collision(); a.streaming(); a.collision(); a.streaming()
         a.collision(); a.streaming()
