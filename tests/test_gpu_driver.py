@@ -50,9 +50,11 @@ def test_driver_regression_deck_and_decomposition_independence(tmp_path):
     assert "3D Taylor-Green Vortex" in out and "Re    = 1600" in out
     _invariants(ref)
     assert ke_matches_reference([c[5] for c in ref])
-    for n in (1, 2, 4, 8):
-        got, out = _run(tmp_path, IMEXLBM_MODE="tiled", IMEXLBM_TILES=n)
+    for n, whole_x in ((1, 0), (2, 0), (4, 0), (8, 0), (8, 1)):
+        got, out = _run(tmp_path, IMEXLBM_MODE="tiled", IMEXLBM_TILES=n, IMEXLBM_WHOLE_X=whole_x)
         assert "rx=" in out
+        if whole_x:
+            assert "rx=1" in out
         _invariants(got)
         for a, b in zip(ref, got):
             assert abs(b[1] - a[1]) / a[1] <= TOL_RANK_AGREE
