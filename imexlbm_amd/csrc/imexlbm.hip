@@ -252,7 +252,7 @@ int layout_boxes(imex_ctx *c)
                 b.gl[a] = gl[a];
                 b.tab[a] = I.tab[a];
                 if (n[a] < 1) return fail(IMEX_EINVAL, "more tiles than cells along an axis");
-                if (c->tiled && n[a] < c->halo_depth)
+                if (c->tiled && n[a] < c->halo_depth && !(a == 0 && c->whole_x))   // whole_x: no x ghosts
                     return fail(IMEX_EINVAL, "tile thinner than the halo depth");
             }
             b.nx = n[0]; b.ny = n[1]; b.nz = n[2];
