@@ -582,6 +582,18 @@ bool ts_allowed(const imex_ctx *c, int tile)
     return true;
 }
 
+// Prefetch mode of the two-step pass: 0 none, 1 every ring cell's
+// populations in registers, 2 the partial prefetch (2-byte formats, tiles
+// whose every thread owns a ring cell: 0, 1, 2 and 4; elsewhere 2 means 1).
+// A negative request picks 2 for 2-byte formats and 1 for the others.
+int ts_pf_for(const imex_ctx *c, int tile, int want)
+{
+    const bool partial_ok = c->ssize == 2 && tile != 3;
+    if (want < 0) return partial_ok ? 2 : 1;
+    if (want >= 2) return partial_ok ? 2 : 1;
+    return want;
+}
+
 // Cells the pass's column tiles cover over the cells of the boxes (1 when
 // the tile divides every box).
 double ts_cover(const imex_ctx *c, int tile)
@@ -662,6 +674,20 @@ int launch_ts(imex_ctx *c, int src, int dst)
     const int npb = (int)c->boxes.size(), zseg = std::max(1, c->ts_zseg);
 #define TS(W, P, G) hipLaunchKernelGGL((k_step2<S, W, TX, TY, NTH, true, P, G>), g, b, 0, c->stream, sp, dp, \
                                        c->d_boxes, c->d_pass, npb, c->ts_xcds, zseg)
+    // partial prefetch where every thread owns a ring cell (2-byte formats)
+    constexpr bool PARTIAL = sizeof(S) == 2 && NTH <= (TX + 2) * (TY + 2);
+    if constexpr (PARTIAL) {
+        if (c->ts_pf == 2) {
+#define TSP(W, G) hipLaunchKernelGGL((k_step2<S, W, TX, TY, NTH, true, false, G, PARTIAL_PD>), g, b, 0, c->stream, \
+                                     sp, dp, c->d_boxes, c->d_pass, npb, c->ts_xcds, zseg)
+            if (c->tiled) TSP(false, true);
+            else if (c->ts_buf) TSP(true, false);
+            else TSP(true, true);
+#undef TSP
+            HIPC(hipGetLastError());
+            return IMEX_OK;
+        }
+    }
     if (c->tiled) {
         if (c->ts_pf) TS(false, true, true); else TS(false, false, true);
     } else if (c->ts_buf) {
@@ -1147,7 +1173,7 @@ int imex_create(imex_ctx **out, const imex_config *cfg_in, const imex_domain *do
         }
         c->ts_zseg = cfg.two_step_zseg > 0 ? cfg.two_step_zseg : (zdef > 0 ? zdef : 16);
         c->ts_xcds = cfg.two_step_xcds > 0 ? cfg.two_step_xcds : 8;
-        c->ts_pf = cfg.two_step_prefetch == 1 || (cfg.two_step_prefetch < 0 && c->ssize >= 4) ? 1 : 0;
+        c->ts_pf = ts_pf_for(c, c->ts_tile, cfg.two_step_prefetch);
         c->ts_buf = cfg.two_step_io == 0 && !c->tiled ? 1 : 0;
     }
     if (c->tiled) build_halo(c);
@@ -1488,8 +1514,7 @@ int imex_autotune(imex_ctx *c, int reps, double *best_ms)
         struct Cand2 { int tile, zseg, xcds, pf, buf; };
         std::vector<Cand2> c2;
         const int nz = c->boxes.empty() ? 0 : c->boxes[0].nz;
-        const int pf_max = c->cfg.two_step_prefetch != 0 ? 1 : 0;
-        const int pf_min = c->cfg.two_step_prefetch == 1 ? 1 : 0;
+        const int want_pf = c->cfg.two_step_prefetch;
         const int buf_max = (!c->tiled && c->cfg.two_step_io != 1) ? 1 : 0;
         const int buf_min = (!c->tiled && c->cfg.two_step_io == 0) ? 1 : 0;
         for (int tile : {0, 1, 2, 3, 4}) {
@@ -1499,9 +1524,13 @@ int imex_autotune(imex_ctx *c, int reps, double *best_ms)
             for (int div : {1, 2, 4, 8, 16})
                 if (nz / div >= std::min(16, nz) && std::find(zs.begin(), zs.end(), nz / div) == zs.end())
                     zs.push_back(nz / div);
+            // prefetch modes: the requested one, or all that the tile builds
+            std::vector<int> pfs;
+            if (want_pf >= 0) pfs.push_back(ts_pf_for(c, tile, want_pf));
+            else for (int pf = 0; pf <= ts_pf_for(c, tile, -1); ++pf) pfs.push_back(pf);
             for (int z : zs)
                 for (int x : {1, 8})
-                    for (int pf = pf_min; pf <= pf_max; ++pf)
+                    for (int pf : pfs)
                         for (int buf = buf_min; buf <= buf_max; ++buf) c2.push_back({tile, z, x, pf, buf});
         }
         const int keep[6] = {c->two_step, c->ts_tile, c->ts_zseg, c->ts_xcds, c->ts_pf, c->ts_buf};

@@ -456,6 +456,13 @@ __device__ inline void lds_pull_cell(const typename LdsWord<S>::type *gm, const 
      ...);
 }
 
+// PD > 0 (2-byte formats): partial prefetch.  Only the first PD populations
+// of each thread's first ring cell of plane p + 2 are loaded ahead, held as
+// stored (a 2-byte value widened at once would make the wave wait for its
+// load right there).  That fits the registers that a full prefetch of a
+// 2-byte pass exceeds, and still takes PD/27 of the first step's load burst
+// off the time between the barriers.
+//
 // PF: prefetch.  The populations of plane p + 2 are loaded into registers
 // before the barrier and the second step of plane p, and collided in the next
 // iteration, so the HBM latency of the first step hides behind the second
@@ -471,7 +478,28 @@ struct PassBox { int wg0, box, tiles_x, ntile; };
 // the box, and the ring and the planes z0 - 1 and z1 are ghost cells, filled
 // two layers deep (with the directions each layer is read for) before the
 // pass; a box that keeps x whole (wrap_x) wraps the ring in x instead.
-template <class S, bool WRAP, int TX, int TY, int NTH, bool NT, bool PF, bool GIO = true>
+// Pulled populations PD..26 of a cell (the first PD come from a prefetch).
+template <class S, int PD, class PL, int... I>
+__device__ inline void load_cell_tail(const PL &P, int dsb, int xy, const Nbr &n,
+                                      typename Store<S>::real (&f)[Q], std::integer_sequence<int, I...>)
+{
+    ((I >= PD ? (void)(f[I] = P.template ld<EZ[I]>((xy + pull_xy<I>(n)) * (int)sizeof(S), I * dsb)) : (void)0), ...);
+}
+
+// Pulled populations 0..PD-1 of a cell of a 2-byte format, as stored.
+template <int PD, class PL, int... I>
+__device__ inline void load_cell_head_raw(const PL &P, int dsb, int xy, const Nbr &n, unsigned short (&h)[PD],
+                                          std::integer_sequence<int, I...>)
+{
+    ((h[I] = P.template raw<EZ[I]>((xy + pull_xy<I>(n)) * 2, I * dsb)), ...);
+}
+
+// Directions of the partial prefetch of 2-byte formats (PD): the first 11
+// populations of each thread's first ring cell.  More made no further gain at
+// 512^3; every thread owns a ring cell in the tiles that use it.
+constexpr int PARTIAL_PD = 11;
+
+template <class S, bool WRAP, int TX, int TY, int NTH, bool NT, bool PF, bool GIO = true, int PD = 0>
 __global__ void __launch_bounds__(NTH)
 k_step2(const S *__restrict__ src, S *__restrict__ dst, const BoxDesc *__restrict__ boxes,
         const PassBox *__restrict__ pb, int npb, int xcds, int zseg)
@@ -495,8 +523,8 @@ k_step2(const S *__restrict__ src, S *__restrict__ dst, const BoxDesc *__restric
     S *dbox = dst + d.base + d.zorg;
     const int dsb = (int)(d.dstride * (long long)sizeof(S));
 
-    // ring cell q of plane p: loads its pulled populations into f
-    auto load_ring = [&](int p, int q, R (&f)[Q]) {
+    // ring cell q of plane p: its source planes, in-plane offset and neighbours
+    auto ring_at = [&](int p, int q, int &xy, Nbr &n) {
         int z = p, zm = p - 1, zp = p + 1;
         int x = x0 - 1 + q % RX, y = y0 - 1 + q / RX;
         if constexpr (WRAP) {
@@ -508,10 +536,17 @@ k_step2(const S *__restrict__ src, S *__restrict__ dst, const BoxDesc *__restric
         } else if (d.wrap_x) {
             if (x < 0) x += d.nx; else if (x >= d.nx) x -= d.nx;
         }
-        const Planes<S, GIO> P(sbox + (long long)zm * d.sz, sbox + (long long)z * d.sz,
-                               sbox + (long long)zp * d.sz, dbox);
-        const Nbr n = neighbours<WRAP>(d, x, y, z);
-        load_cell_io<S, true>(P, dsb, d.org_xy + x + y * d.sy, n, f, seq);
+        n = neighbours<WRAP>(d, x, y, z);
+        xy = d.org_xy + x + y * d.sy;
+        return Planes<S, GIO>(sbox + (long long)zm * d.sz, sbox + (long long)z * d.sz,
+                              sbox + (long long)zp * d.sz, dbox);
+    };
+    // loads the pulled populations of ring cell q of plane p into f
+    auto load_ring = [&](int p, int q, R (&f)[Q]) {
+        int xy;
+        Nbr n;
+        const auto P = ring_at(p, q, xy, n);
+        load_cell_io<S, true>(P, dsb, xy, n, f, seq);
     };
     // collide ring cell q of the intermediate plane p and keep it in LDS
     auto put_ring = [&](int p, int q, R (&f)[Q]) {
@@ -545,7 +580,39 @@ k_step2(const S *__restrict__ src, S *__restrict__ dst, const BoxDesc *__restric
     };
     first(z0 - 1);
     first(z0);
-    if constexpr (PF) {
+    if constexpr (PD > 0) {
+        static_assert(sizeof(S) == 2 && NTH <= RC && !PF, "partial prefetch: 2-byte formats, every thread owns a ring cell");
+        unsigned short pre[PD];
+        auto fetch_head = [&](int p) {
+            int xy;
+            Nbr n;
+            const auto P = ring_at(p, (int)threadIdx.x, xy, n);
+            load_cell_head_raw<PD>(P, dsb, xy, n, pre, std::make_integer_sequence<int, PD>{});
+        };
+        fetch_head(z0 + 1);
+#pragma clang loop unroll(disable)
+        for (int k = z0; k < z1; ++k) {
+            {
+                int xy;
+                Nbr n;
+                const auto P = ring_at(k + 1, (int)threadIdx.x, xy, n);
+                R f[Q];
+                load_cell_tail<S, PD>(P, dsb, xy, n, f, seq);
+#pragma unroll
+                for (int ii = 0; ii < PD; ++ii) f[ii] = widen16<S>(pre[ii]);
+                put_ring(k + 1, (int)threadIdx.x, f);
+            }
+            for (int q = (int)threadIdx.x + NTH; q < RC; q += NTH) {
+                R f[Q];
+                load_ring(k + 1, q, f);
+                put_ring(k + 1, q, f);
+            }
+            if (k + 1 < z1) fetch_head(k + 2);
+            __syncthreads();
+            second(k);
+            __syncthreads();
+        }
+    } else if constexpr (PF) {
         R pf[NC][Q];
 #pragma unroll
         for (int c = 0; c < NC; ++c)

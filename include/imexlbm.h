@@ -109,8 +109,12 @@ typedef struct imex_config {
                              XCDs, as xcd_groups; 0 = auto (8: each XCD
                              sweeps its own z-range)                     */
     int two_step_prefetch; /* load the next plane's populations into
-                             registers ahead of the barrier; 1 = on, 0 =
-                             off, -1 = auto (on for 8- and 4-byte formats) */
+                             registers ahead of the barrier: 1 = every
+                             ring cell's, 2 = the first 11 of each
+                             thread's first ring cell (2-byte formats,
+                             tiles 0, 1, 2 and 4; elsewhere as 1), 0 =
+                             off, -1 = auto (2 for 2-byte formats, 1 for
+                             the others; the autotuner tries each)       */
     int two_step_io;      /* periodic boxes: the pass's loads and stores
                              through 1 = global addresses, 0 = buffer
                              descriptors (direction offset in a scalar

@@ -42,9 +42,11 @@ def _pair(d, prec, steps, **two):
     return fa, fb, ca, cb
 
 
+# pf 2: the partial prefetch of 2-byte formats (tile 3, whose threads
+# outnumber its ring cells, falls back to the full one)
 CASES = ([("fp64", t, pf) for t in (2, 3) for pf in (0, 1)] +
          [("fp32", t, pf) for t in (0, 1, 2, 3) for pf in (0, 1)] +
-         [(p, t, pf) for p in ("fp16", "bf16") for t in (0, 1, 2, 3, 4) for pf in (0, 1)])
+         [(p, t, pf) for p in ("fp16", "bf16") for t in (0, 1, 2, 3, 4) for pf in (0, 1, 2)])
 
 
 @pytest.mark.parametrize("prec,tile,pf", CASES)
