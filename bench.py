@@ -78,7 +78,8 @@ def run_mode(capi, prec, n, re, u0, steps, warmup, autotune, use_graph, mode="wr
         "roofline_mlups": round(HBM_PEAK_GBS * 1e9 / bpu / 1e6, 1),
         "launch": {k: info[k] for k in ("block", "cells_per_thread", "persistent", "nontemporal",
                                         "dir_pad", "use_graph", "xcd_groups", "xcd_run", "io16",
-                                        "two_step", "two_step_tile", "two_step_zseg", "two_step_xcds")},
+                                        "two_step", "two_step_tile", "two_step_zseg", "two_step_xcds",
+                                        "two_step_prefetch", "two_step_io")},
         "updates_per_launch": 2 if info["two_step"] else 1,
         "autotune_best_ms": None if tuned_ms is None else round(tuned_ms, 4),
         "mass": cons[0], "ke": cons[4],
