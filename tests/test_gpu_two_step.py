@@ -115,9 +115,10 @@ def test_deferred_collision_streaming_pairs(prec):
         assert np.array_equal(a.get_f(), b.get_f())
 
 
-def test_two_step_graph_replay():
+@pytest.mark.parametrize("prec", ["fp32", "fp16"])   # fp16: the partial prefetch
+def test_two_step_graph_replay(prec):
     d = _dom(64, nz=12)
-    with _ctx(d, precision="fp32", two_step=1, use_graph=1) as a, _ctx(d, precision="fp32", two_step=0) as b:
+    with _ctx(d, precision=prec, two_step=1, use_graph=1) as a, _ctx(d, precision=prec, two_step=0) as b:
         a.initialize(); b.initialize()
         a.step(1); b.step(1)
         for k in range(4):
