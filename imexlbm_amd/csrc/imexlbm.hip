@@ -669,6 +669,49 @@ int ts_default_zseg(const imex_ctx *c, int tile)
     return ts_grid(c, tile, z) >= c->ncu ? z : 0;
 }
 
+// Planes per sweep tried for a tile (by the default choice and the
+// autotuner): the rule above, and nz, nz/2, ..., nz/16 down to 16 planes.
+std::vector<int> ts_zseg_candidates(const imex_ctx *c, int tile)
+{
+    int nz = 1;
+    for (const BoxDesc &b : c->boxes) nz = std::max(nz, b.nz);
+    std::vector<int> zs;
+    if (const int z = ts_default_zseg(c, tile)) zs.push_back(z);
+    for (int div : {1, 2, 4, 8, 16})
+        if (nz / div >= std::min(16, nz) && std::find(zs.begin(), zs.end(), nz / div) == zs.end())
+            zs.push_back(nz / div);
+    return zs;
+}
+
+// Modelled duration of a two-step pass: waves of workgroups x planes of the
+// longest sweep (two of them recomputed) x the plane time of the tile.  The
+// plane times (microseconds per plane for one workgroup, every CU busy) and
+// the workgroups that fit a CU (LDS) were measured at 512^3 on MI355X.  The
+// wave count prices the idle tail of a pass whose workgroups do not fill its
+// last wave, the main reason one sweep length or tile beats another at sizes
+// other than powers of two.  Only the default choice uses it; the autotuner
+// measures.
+double ts_model(const imex_ctx *c, int tile, int zseg)
+{
+    static const double us2[5] = {7.23, 8.53, 9.08, 10.5, 6.37};    // 2-byte formats, tiles 0..4
+    static const double us4[4] = {5.98, 6.29, 6.56, 7.12};          // FP32
+    static const double us8[4] = {0.0, 0.0, 6.66, 6.15};            // FP64 (tiles 2, 3)
+    static const int wpc2[5] = {2, 2, 4, 4, 1}, wpc4[4] = {1, 1, 2, 2};
+    const double us = c->ssize == 2 ? us2[tile] : (c->ssize == 4 ? us4[tile] : us8[tile]);
+    const int wpc = c->ssize == 2 ? wpc2[tile] : (c->ssize == 4 ? wpc4[tile] : 1);
+    int tx, ty;
+    ts_dims(tile, tx, ty);
+    long long g = 0;
+    int zl = 0;
+    for (const BoxDesc &b : c->boxes) {
+        const int z = std::max(1, std::min(zseg, b.nz));
+        zl = std::max(zl, z);
+        g += (long long)((b.nx + tx - 1) / tx) * ((b.ny + ty - 1) / ty) * ((b.nz + z - 1) / z);
+    }
+    const long long slots = (long long)c->ncu * wpc;
+    return (double)((g + slots - 1) / slots) * (zl + 2) * us;
+}
+
 template <class S, int TX, int TY, int NTH>
 int launch_ts(imex_ctx *c, int src, int dst)
 {
@@ -1156,15 +1199,47 @@ int imex_create(imex_ctx **out, const imex_config *cfg_in, const imex_domain *do
     if (r != IMEX_OK) return bail(r);
     build_tiles(c);
     {
-        // default tile: the fewest covered cells, ties in this order
+        // default shape: the tile and sweep length of least modelled time among
+        // those with a workgroup for nine CUs in ten (any allowed one when two
+        // steps per pass are requested); without one, the tile covering the
+        // fewest cells is reported.  Sweep lengths: the candidates the
+        // autotuner tries and the balanced ones, ceil(nz/k), down to 16
+        // planes.  Those of the tiles other than 64x16 are kept to the length
+        // rule's start (nz/4, at least 64 planes, for 4- and 8-byte formats;
+        // nz/8, at least 32, for 2-byte ones): longer ones measured slower at
+        // 512^3 than the model says.
         c->ts_tile = cfg.two_step_tile >= 0 ? cfg.two_step_tile : (c->ssize == 8 ? 3 : (c->ssize == 2 ? 4 : 0));
         if (cfg.two_step_tile < 0) {
             double best = 1e30;
             for (int t : {4, 0, 1, 3, 2})
                 if (ts_allowed(c, t) && ts_cover(c, t) < best - 1e-9) { best = ts_cover(c, t); c->ts_tile = t; }
         }
+        int zdef = 0;
+        for (int pass = 0; pass < 2 && zdef == 0; ++pass) {
+            if (pass == 1 && cfg.two_step != 1) break;
+            double best = 1e300;
+            for (int t : {4, 0, 1, 3, 2}) {
+                if ((cfg.two_step_tile >= 0 && t != cfg.two_step_tile) || !ts_allowed(c, t)) continue;
+                int nz = 1;
+                for (const BoxDesc &b : c->boxes) nz = std::max(nz, b.nz);
+                std::vector<int> zs = cfg.two_step_zseg > 0 ? std::vector<int>{cfg.two_step_zseg}
+                                                            : ts_zseg_candidates(c, t);
+                if (cfg.two_step_zseg <= 0)
+                    for (int k = 1; (nz + k - 1) / k >= std::min(16, nz); ++k) {
+                        const int z = (nz + k - 1) / k;
+                        if (std::find(zs.begin(), zs.end(), z) == zs.end()) zs.push_back(z);
+                        if (z == 1) break;
+                    }
+                const int zcap = t == 4 ? nz : std::min(nz, c->ssize == 2 ? std::max(32, nz / 8) : std::max(64, nz / 4));
+                for (int z : zs) {
+                    if ((pass == 0 && 10 * ts_grid(c, t, z) < 9LL * c->ncu) || (cfg.two_step_zseg <= 0 && z > zcap))
+                        continue;
+                    const double m = ts_model(c, t, z);
+                    if (m < best - 1e-9) { best = m; c->ts_tile = t; zdef = z; }
+                }
+            }
+        }
         const bool ok = ts_allowed(c, c->ts_tile);
-        const int zdef = ok ? ts_default_zseg(c, c->ts_tile) : 0;
         if (cfg.two_step == 1 && !ok)
             return bail(fail(IMEX_EINVAL, "two_step needs boxes whose nx and ny the column tile divides (tiled: "
                                           "boxes at least 2 planes deep; batches: one launch for all; FP64: 32x8 "
@@ -1518,17 +1593,14 @@ int imex_autotune(imex_ctx *c, int reps, double *best_ms)
     if (c->cfg.two_step != 0) {
         struct Cand2 { int tile, zseg, xcds, pf, buf; };
         std::vector<Cand2> c2;
-        const int nz = c->boxes.empty() ? 0 : c->boxes[0].nz;
         const int want_pf = c->cfg.two_step_prefetch;
         const int buf_max = (!c->tiled && c->cfg.two_step_io != 1) ? 1 : 0;
         const int buf_min = (!c->tiled && c->cfg.two_step_io == 0) ? 1 : 0;
         for (int tile : {0, 1, 2, 3, 4}) {
             if (!ts_allowed(c, tile)) continue;
-            std::vector<int> zs;
-            if (const int z = ts_default_zseg(c, tile)) zs.push_back(z);
-            for (int div : {1, 2, 4, 8, 16})
-                if (nz / div >= std::min(16, nz) && std::find(zs.begin(), zs.end(), nz / div) == zs.end())
-                    zs.push_back(nz / div);
+            std::vector<int> zs = ts_zseg_candidates(c, tile);
+            // the shape in use (the modelled default or an earlier pick) is a candidate too
+            if (tile == c->ts_tile && std::find(zs.begin(), zs.end(), c->ts_zseg) == zs.end()) zs.push_back(c->ts_zseg);
             // prefetch modes: the requested one, or all that the tile builds
             std::vector<int> pfs;
             if (want_pf >= 0) pfs.push_back(ts_pf_for(c, tile, want_pf));
