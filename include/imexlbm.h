@@ -101,9 +101,10 @@ typedef struct imex_config {
     int two_step_tile;    /* column tile: 0 = 64x8 cells (512 threads),
                              1 = 32x16 (512), 2 = 32x8 (256), 3 = 32x8
                              (384), 4 = 64x16 (1024, 2-byte formats only);
-                             -1 = auto (the fewest cells covered; ties:
-                             4 for 2-byte formats, 0 for FP32, 3 for
-                             FP64; FP64 builds only 2 and 3)              */
+                             FP64 builds only 2 and 3.  -1 = auto: with
+                             two_step_zseg, the shape of least modelled
+                             pass time (workgroup waves x planes per
+                             sweep x the tile's measured plane time)     */
     int two_step_zseg;    /* z-planes per workgroup sweep; 0 = auto       */
     int two_step_xcds;    /* tile order of the two-step pass over the
                              XCDs, as xcd_groups; 0 = auto (8: each XCD
