@@ -147,7 +147,7 @@ struct imex_ctx {
     double *h_out = nullptr;
     double *d_stage = nullptr;
     size_t stage_cap = 0;
-    void *pool[2] = {nullptr, nullptr};
+    void *pool[3] = {nullptr, nullptr, nullptr};   // [2]: the autotuner's temporary pool
     long long pool_elems = 0;
     void *xbuf = nullptr;
     long long xbuf_elems = 0;
@@ -1554,11 +1554,49 @@ int imex_autotune(imex_ctx *c, int reps, double *best_ms)
         build_tiles(c);
         return upload_tiles(c);
     };
-    // fused launches cur -> other without swapping: the state is untouched
+    // Timed launches never write the live pool, so the state is untouched.
+    // Launches from the live pool into the other one, repeated, would keep a
+    // pool of up to ~250 MB in the 256 MB Infinity Cache, which a run does not
+    // (it alternates two pools): the timings would be optimistic at those
+    // sizes.  Up to 512 MB a temporary third pool, a copy of the live one,
+    // is allocated, and the launches alternate between it and the other pool.
+    struct TmpPool {
+        imex_ctx *c;
+        void *p = nullptr;
+        ~TmpPool()
+        {
+            if (!p) return;
+            (void)hipStreamSynchronize(c->stream);
+            (void)hipFree(p);
+            c->pool[2] = nullptr;
+        }
+    } tmp{c};
+    {
+        const size_t pool_bytes = (size_t)c->pool_elems * c->ssize;
+        if (pool_bytes <= (512ull << 20)) {
+            if (hipMalloc(&tmp.p, pool_bytes) == hipSuccess &&
+                hipMemcpyAsync(tmp.p, c->pool[c->cur], pool_bytes, hipMemcpyDeviceToDevice, c->stream) == hipSuccess)
+                c->pool[2] = tmp.p;
+            else
+                (void)hipGetLastError();
+        }
+    }
+    // source and destination pool of timed launch r (-1: the warm-up one)
+    auto route = [&](int r, int &s, int &d) {
+        if (!c->pool[2]) { s = c->cur; d = 1 - c->cur; return; }
+        const bool a = r < 0 || (r % 2) == 1;
+        s = a ? 2 : 1 - c->cur;
+        d = a ? 1 - c->cur : 2;
+    };
     auto time_it = [&](int n, double &ms_out) -> int {
-        TRY(launch_step(c, OP_FUSED, c->cur, 1 - c->cur));
+        int s, d;
+        route(-1, s, d);
+        TRY(launch_step(c, OP_FUSED, s, d));
         HIPC(hipEventRecord(c->ev[0], c->stream));
-        for (int r = 0; r < n; ++r) TRY(launch_step(c, OP_FUSED, c->cur, 1 - c->cur));
+        for (int r = 0; r < n; ++r) {
+            route(r, s, d);
+            TRY(launch_step(c, OP_FUSED, s, d));
+        }
         HIPC(hipEventRecord(c->ev[1], c->stream));
         HIPC(hipEventSynchronize(c->ev[1]));
         float ms = 0.f;
@@ -1616,9 +1654,14 @@ int imex_autotune(imex_ctx *c, int reps, double *best_ms)
         for (size_t k = 0; k < c2.size(); ++k) {
             c->ts_tile = c2[k].tile; c->ts_zseg = c2[k].zseg; c->ts_xcds = c2[k].xcds; c->ts_pf = c2[k].pf;
             c->ts_buf = c2[k].buf;
-            TRY(launch_step2(c, c->cur, 1 - c->cur));
+            int s, d;
+            route(-1, s, d);
+            TRY(launch_step2(c, s, d));
             HIPC(hipEventRecord(c->ev[0], c->stream));
-            for (int r = 0; r < reps; ++r) TRY(launch_step2(c, c->cur, 1 - c->cur));
+            for (int r = 0; r < reps; ++r) {
+                route(r, s, d);
+                TRY(launch_step2(c, s, d));
+            }
             HIPC(hipEventRecord(c->ev[1], c->stream));
             HIPC(hipEventSynchronize(c->ev[1]));
             float ms = 0.f;
