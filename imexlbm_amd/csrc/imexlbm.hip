@@ -1204,10 +1204,9 @@ int imex_create(imex_ctx **out, const imex_config *cfg_in, const imex_domain *do
         // steps per pass are requested); without one, the tile covering the
         // fewest cells is reported.  Sweep lengths: the candidates the
         // autotuner tries and the balanced ones, ceil(nz/k), down to 16
-        // planes.  Those of the tiles other than 64x16 are kept to the length
-        // rule's start (nz/4, at least 64 planes, for 4- and 8-byte formats;
-        // nz/8, at least 32, for 2-byte ones): longer ones measured slower at
-        // 512^3 than the model says.
+        // planes.  Those of the tiles other than 64x16 are capped (FP64 nz/4
+        // and FP32 nz/8, at least 64 planes; 2-byte formats nz/8, at least
+        // 32): longer ones measured slower at 512^3 than the model says.
         c->ts_tile = cfg.two_step_tile >= 0 ? cfg.two_step_tile : (c->ssize == 8 ? 3 : (c->ssize == 2 ? 4 : 0));
         if (cfg.two_step_tile < 0) {
             double best = 1e30;
@@ -1230,7 +1229,9 @@ int imex_create(imex_ctx **out, const imex_config *cfg_in, const imex_domain *do
                         if (std::find(zs.begin(), zs.end(), z) == zs.end()) zs.push_back(z);
                         if (z == 1) break;
                     }
-                const int zcap = t == 4 ? nz : std::min(nz, c->ssize == 2 ? std::max(32, nz / 8) : std::max(64, nz / 4));
+                const int zcap = t == 4 ? nz
+                                        : std::min(nz, c->ssize == 2 ? std::max(32, nz / 8)
+                                                                     : std::max(64, nz / (c->ssize == 4 ? 8 : 4)));
                 for (int z : zs) {
                     if ((pass == 0 && 10 * ts_grid(c, t, z) < 9LL * c->ncu) || (cfg.two_step_zseg <= 0 && z > zcap))
                         continue;
