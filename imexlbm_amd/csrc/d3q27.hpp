@@ -214,20 +214,27 @@ __host__ __device__ constexpr float weight3f(int ii)
     return ii <= 6 ? W3_FACE : (ii <= 18 ? W3_EDGE : W3_CORNER);
 }
 
-__device__ inline void collide32(float (&g)[Q], float omega)
+// one: 1.0.  Each add of two populations, and each subtraction of one, is
+// written as a fused multiply-add by one, which rounds exactly like the add.
+// With the literal 1.0 the compiler emits the plain add.  Passed in a register
+// it cannot see through (FP16 storage), a widened FP16 operand is read by
+// v_fma_mix_f32 in place, and the 27 widening conversions per cell go.
+__device__ inline void collide32(float (&g)[Q], float omega, float one = 1.0f)
 {
-    const float xp = (((g[1] + g[7]) + (g[9] + g[11])) + ((g[13] + g[19]) + (g[21] + g[23]))) + g[25];
-    const float xm = (((g[2] + g[8]) + (g[10] + g[12])) + ((g[14] + g[20]) + (g[22] + g[24]))) + g[26];
-    const float x0 = (((g[0] + g[3]) + (g[4] + g[5])) + ((g[6] + g[15]) + (g[16] + g[17]))) + g[18];
-    const float yp = (((g[3] + g[7]) + (g[10] + g[15])) + ((g[17] + g[19]) + (g[22] + g[23]))) + g[26];
-    const float ym = (((g[4] + g[8]) + (g[9] + g[16])) + ((g[18] + g[20]) + (g[21] + g[24]))) + g[25];
-    const float zp = (((g[5] + g[11]) + (g[14] + g[15])) + ((g[18] + g[19]) + (g[21] + g[24]))) + g[26];
-    const float zm = (((g[6] + g[12]) + (g[13] + g[16])) + ((g[17] + g[20]) + (g[22] + g[23]))) + g[25];
+    auto add = [one](float a, float b) { return __builtin_fmaf(a, one, b); };
+    auto sub = [one](float a, float b) { return __builtin_fmaf(b, -one, a); };   // a - b
+    const float xp = add(g[25], (add(g[1], g[7]) + add(g[9], g[11])) + (add(g[13], g[19]) + add(g[21], g[23])));
+    const float xm = add(g[26], (add(g[2], g[8]) + add(g[10], g[12])) + (add(g[14], g[20]) + add(g[22], g[24])));
+    const float x0 = add(g[18], (add(g[0], g[3]) + add(g[4], g[5])) + (add(g[6], g[15]) + add(g[16], g[17])));
+    const float yp = add(g[26], (add(g[3], g[7]) + add(g[10], g[15])) + (add(g[17], g[19]) + add(g[22], g[23])));
+    const float ym = add(g[25], (add(g[4], g[8]) + add(g[9], g[16])) + (add(g[18], g[20]) + add(g[21], g[24])));
+    const float zp = add(g[26], (add(g[5], g[11]) + add(g[14], g[15])) + (add(g[18], g[19]) + add(g[21], g[24])));
+    const float zm = add(g[25], (add(g[6], g[12]) + add(g[13], g[16])) + (add(g[17], g[20]) + add(g[22], g[23])));
     const float sig = (xp + xm) + x0;
     const float u = xp - xm, v = yp - ym, w = zp - zm;
     const float udu = __builtin_fmaf(u, u, __builtin_fmaf(v, v, w * w));
     const float A = __builtin_fmaf(-1.5f, udu, sig);
-    g[0] = __builtin_fmaf(omega, (float)W_REST * A - g[0], g[0]);
+    g[0] = __builtin_fmaf(omega, sub((float)W_REST * A, g[0]), g[0]);
     const float s[13] = {u, v, w, u + v, u - v, u + w, u - w, v + w, v - w,
                          (u + v) + w, (u - v) + w, (u + v) - w, (u - v) - w};
     // w*A once per weight group; the symmetric part of the pair is then
@@ -239,8 +246,8 @@ __device__ inline void collide32(float (&g)[Q], float omega)
         const int grp = a <= 6 ? 0 : (a <= 18 ? 1 : 2);
         const float B = __builtin_fmaf(s[pr] * (float)(4.5 * weight(a)), s[pr], wA[grp]);
         const float C = s[pr] * weight3f(a);
-        g[a] = __builtin_fmaf(omega, (B + C) - g[a], g[a]);
-        g[b] = __builtin_fmaf(omega, (B - C) - g[b], g[b]);
+        g[a] = __builtin_fmaf(omega, sub(B + C, g[a]), g[a]);
+        g[b] = __builtin_fmaf(omega, sub(B - C, g[b]), g[b]);
     }
 }
 

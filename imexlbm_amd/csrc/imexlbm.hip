@@ -586,13 +586,15 @@ bool ts_allowed(const imex_ctx *c, int tile)
 // populations in registers, 2 the partial prefetch (2-byte formats, tiles
 // whose every thread owns a ring cell: 0, 1, 2 and 4; elsewhere 2 means 1).
 // A negative request picks 2 for 2-byte formats and 1 for the others, except
-// for the 64x16 tile over ghost cells, whose 1024 threads leave 128 registers
-// each: there any prefetch spills, so it runs without.
+// for BF16 in the 64x16 tile over ghost cells, whose 1024 threads leave 128
+// registers each: there the prefetch spills, so it runs without.  (FP16 fits:
+// its collision over ghost-bounded boxes reads the halves without widening
+// them first.)
 int ts_pf_for(const imex_ctx *c, int tile, int want)
 {
     const bool partial_ok = c->ssize == 2 && tile != 3;
     if (want < 0) {
-        if (c->ssize == 2 && tile == 4 && c->tiled) return 0;
+        if (c->prec == IMEX_BF16 && tile == 4 && c->tiled) return 0;
         return partial_ok ? 2 : 1;
     }
     if (want >= 2) return partial_ok ? 2 : 1;

@@ -17,6 +17,7 @@
 // one launch per step.  Wavefronts own 64 consecutive x cells, so each of the
 // 27 loads and 27 stores of a wavefront is one contiguous 64-element run.
 #pragma once
+#include <type_traits>
 #include <utility>
 
 #include "d3q27.hpp"
@@ -181,6 +182,25 @@ __device__ inline int pull_xy(const Nbr &n)
     if constexpr (EX[ii] == 1) o += n.xm; else if constexpr (EX[ii] == -1) o += n.xp;
     if constexpr (EY[ii] == 1) o += n.ym; else if constexpr (EY[ii] == -1) o += n.yp;
     return o;
+}
+
+// The one of collide32.  For FP16 storage in kernels over ghost-bounded
+// boxes (!WRAP) it is a 1.0 the compiler cannot fold (an SGPR set by
+// s_mov_b32): the collision then reads widened halves through v_fma_mix_f32,
+// which drops the 27 widening conversions and about 25 registers per thread.
+// Those kernels are register-bound (the 64x16 two-step tile spilled without
+// it) and run 10-25% faster at 256^3; the periodic kernels ran about 3%
+// slower with it at 512^3 and keep the conversions.  Other formats: the
+// literal.
+template <class S, bool WRAP> __device__ inline float collide_one()
+{
+    if constexpr (std::is_same<S, _Float16>::value && !WRAP) {
+        float r;
+        asm("s_mov_b32 %0, 1.0" : "=s"(r));
+        return r;
+    } else {
+        return 1.0f;
+    }
 }
 
 template <class S> __device__ inline float widen16(unsigned short b);
@@ -369,7 +389,10 @@ k_step(const S *__restrict__ src, S *__restrict__ dst, const BoxDesc *__restrict
                     __builtin_amdgcn_sched_barrier(0);
                 }
                 if constexpr (OP != OP_STREAM) {
-                    if constexpr (Store<S>::shifted) { collide32(f1, d.omega_f); collide32(f2, d.omega_f); }
+                    if constexpr (Store<S>::shifted) {
+                        collide32(f1, d.omega_f, collide_one<S, WRAP>());
+                        collide32(f2, d.omega_f, collide_one<S, WRAP>());
+                    }
                     else { collide64(f1, d.omega); collide64(f2, d.omega); }
                 }
                 store_cell_io<S, NT>(P, dsb, xy1, f1, std::make_integer_sequence<int, Q>{});
@@ -386,7 +409,7 @@ k_step(const S *__restrict__ src, S *__restrict__ dst, const BoxDesc *__restrict
             R f[Q];
             load_cell_io<S, GATHER>(P, dsb, xy, n, f, std::make_integer_sequence<int, Q>{});
             if constexpr (OP != OP_STREAM) {
-                if constexpr (Store<S>::shifted) collide32(f, d.omega_f);
+                if constexpr (Store<S>::shifted) collide32(f, d.omega_f, collide_one<S, WRAP>());
                 else collide64(f, d.omega);
             }
             store_cell_io<S, NT>(P, dsb, xy, f, std::make_integer_sequence<int, Q>{});
@@ -551,7 +574,7 @@ k_step2(const S *__restrict__ src, S *__restrict__ dst, const BoxDesc *__restric
     // collide ring cell q of the intermediate plane p and keep it in LDS
     auto put_ring = [&](int p, int q, R (&f)[Q]) {
         Wd *gm = lds, *g0 = lds + (9 + 9 * ((p + 4) & 1)) * RC, *gp = lds + (27 + 9 * ((p + 3) % 3)) * RC;
-        if constexpr (Store<S>::shifted) collide32(f, d.omega_f);
+        if constexpr (Store<S>::shifted) collide32(f, d.omega_f, collide_one<S, WRAP>());
         else collide64(f, d.omega);
         lds_store_cell<S, RC>(gm, g0, gp, q, f, seq);
     };
@@ -573,7 +596,7 @@ k_step2(const S *__restrict__ src, S *__restrict__ dst, const BoxDesc *__restric
             if (WRAP && (x0 + tx >= d.nx || y0 + ty >= d.ny)) continue;   // past a ragged edge
             R f[Q];
             lds_pull_cell<S, RC, RX>(gm, g0, gp, (ty + 1) * RX + tx + 1, f, seq);
-            if constexpr (Store<S>::shifted) collide32(f, d.omega_f);
+            if constexpr (Store<S>::shifted) collide32(f, d.omega_f, collide_one<S, WRAP>());
             else collide64(f, d.omega);
             store_cell_io<S, NT>(P, dsb, d.org_xy + (x0 + tx) + (y0 + ty) * d.sy, f, seq);
         }
