@@ -114,8 +114,10 @@ typedef struct imex_config {
                              ring cell's, 2 = the first 11 of each
                              thread's first ring cell (2-byte formats,
                              tiles 0, 1, 2 and 4; elsewhere as 1), 0 =
-                             off, -1 = auto (2 for 2-byte formats, 1 for
-                             the others; the autotuner tries each)       */
+                             off, -1 = auto (2 for 2-byte formats, but 0
+                             for BF16 in tile 4 over ghost cells, where
+                             it spills; 1 for the others; the autotuner
+                             tries each)                                 */
     int two_step_io;      /* periodic boxes: the pass's loads and stores
                              through 1 = global addresses, 0 = buffer
                              descriptors (direction offset in a scalar
