@@ -196,6 +196,35 @@ def test_tiled_two_step_equals_wrap_single_steps(prec, ntiles, halo, direct, who
         assert abs(ca[k] - cb[k]) <= 1e-12 * abs(cb[k])
 
 
+@pytest.mark.parametrize("prec", ["fp16", "bf16"])
+@pytest.mark.parametrize("two,tile,pf", [(0, None, None), (1, 0, 2), (1, 1, 2), (1, 2, 2), (1, 4, 2),
+                                         (1, 4, 0), (1, 2, 0)])
+def test_tiled_16bit_collision_equals_wrap(prec, two, tile, pf):
+    # over ghost-bounded boxes the FP16 collision reads the stored halves
+    # through mixed-precision FMAs, the periodic kernels widen them first:
+    # the populations must still agree bit for bit, one step per pass and
+    # in every column tile and prefetch mode of the two-step pass (1x2x4
+    # tiles of 64x32x16, which all four tiles divide)
+    from imexlbm_amd import capi
+    d = _dom(64)
+    kw = dict(two_step=two)
+    if tile is not None:
+        kw.update(two_step_tile=tile, two_step_prefetch=pf)
+    with _ctx(d, precision=prec, mode=capi.TILED, ntiles=8, halo_depth=3, whole_x=1, **kw) as a:
+        a.initialize()
+        a.step(7)
+        info = a.info()
+        fa = a.get_f()
+    assert info["two_step"] == two and (info["rx"], info["ry"], info["rz"]) == (1, 2, 4)
+    if tile is not None:
+        assert info["two_step_tile"] == tile and info["two_step_prefetch"] == pf
+    with _ctx(d, precision=prec, two_step=0) as b:
+        b.initialize()
+        b.step(7)
+        fb = b.get_f()
+    assert np.array_equal(fa, fb)
+
+
 def test_two_step_full_size_fp16_512():
     # the bench configuration: conserved sums of the two paths bit-identical
     # after 1 + 4 steps (each context holds 29 GB)
