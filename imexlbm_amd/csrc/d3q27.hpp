@@ -217,8 +217,9 @@ __host__ __device__ constexpr float weight3f(int ii)
 // one: 1.0.  Each add of two populations, and each subtraction of one, is
 // written as a fused multiply-add by one, which rounds exactly like the add.
 // With the literal 1.0 the compiler emits the plain add.  Passed in a register
-// it cannot see through (FP16 storage), a widened FP16 operand is read by
-// v_fma_mix_f32 in place, and the 27 widening conversions per cell go.
+// it cannot see through (the FP16 kernels over ghost-bounded boxes), a widened
+// FP16 operand is read by v_fma_mix_f32 in place, and the 27 widening
+// conversions per cell go.
 __device__ inline void collide32(float (&g)[Q], float omega, float one = 1.0f)
 {
     auto add = [one](float a, float b) { return __builtin_fmaf(a, one, b); };
