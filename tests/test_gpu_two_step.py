@@ -161,6 +161,24 @@ def test_autotune_with_two_step_candidates(prec):
         assert np.array_equal(a.get_f(), b.get_f())
 
 
+@pytest.mark.parametrize("prec,whole_x", [("fp16", 1), ("bf16", 0), ("fp64", 1)])
+def test_tiled_autotune_keeps_state(prec, whole_x):
+    # the autotuner times candidates on scratch pools: the tiled state and
+    # its ghosts must come back untouched, and the chosen shape must still
+    # give the periodic box's populations
+    from imexlbm_amd import capi
+    d = _dom(64)
+    with _ctx(d, precision=prec, mode=capi.TILED, ntiles=8, halo_depth=3, whole_x=whole_x) as a, \
+            _ctx(d, precision=prec, two_step=0) as b:
+        a.initialize(); b.initialize()
+        a.step(3); b.step(3)
+        before = a.get_f()
+        assert a.autotune(reps=2) > 0
+        assert np.array_equal(a.get_f(), before)
+        a.step(9); b.step(9)
+        assert np.array_equal(a.get_f(), b.get_f())
+
+
 def test_two_step_eligibility():
     from imexlbm_amd import capi
     with _ctx(_dom(32), precision="fp64") as c:   # auto: 4 column tiles are too few workgroups
