@@ -1616,14 +1616,17 @@ int imex_autotune(imex_ctx *c, int reps, double *best_ms)
             std::fprintf(stderr, "autotune block=%d cpt=%d xcds=%d run=%d io16=%d: %.4f ms\n", cands[k].block,
                          cands[k].cpt, cands[k].xcds, cands[k].run, cands[k].io16, t[k].first);
     }
-    // confirmation round: the four fastest again, best of both rounds
+    // confirmation round: the four fastest again over longer runs, which
+    // alone decide (a few launches misorder close shapes)
     std::sort(t.begin(), t.end());
     const size_t top = std::min<size_t>(4, t.size());
     for (size_t k = 0; k < top; ++k) {
         TRY(apply(cands[t[k].second]));
-        double again = 0.0;
-        TRY(time_it(reps, again));
-        t[k].first = std::min(t[k].first, again);
+        TRY(time_it(std::max(16, 4 * reps), t[k].first));
+        if (std::getenv("IMEXLBM_AUTOTUNE_LOG"))
+            std::fprintf(stderr, "autotune confirm block=%d cpt=%d xcds=%d run=%d io16=%d: %.4f ms\n",
+                         cands[t[k].second].block, cands[t[k].second].cpt, cands[t[k].second].xcds,
+                         cands[t[k].second].run, cands[t[k].second].io16, t[k].first);
     }
     std::sort(t.begin(), t.begin() + top);
     TRY(apply(cands[t[0].second]));
