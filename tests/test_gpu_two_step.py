@@ -304,7 +304,8 @@ def test_batched_two_step_equals_instances_alone(prec):
             assert np.array_equal(got[k], b.get_f())
 
 
-RAGGED = [("fp64", 3), ("fp64", 2), ("fp32", 0), ("fp32", 1), ("fp16", 0), ("fp16", 3), ("bf16", 1)]
+RAGGED = [("fp64", 3), ("fp64", 2), ("fp32", 0), ("fp32", 1), ("fp16", 0), ("fp16", 3), ("bf16", 1),
+          ("fp16", 4), ("bf16", 4)]
 
 
 @pytest.mark.parametrize("prec,tile", RAGGED)
@@ -312,7 +313,7 @@ RAGGED = [("fp64", 3), ("fp64", 2), ("fp32", 0), ("fp32", 1), ("fp16", 0), ("fp1
 def test_two_step_ragged_column_tiles(prec, tile, shape):
     # the last column tile runs past nx / ny: its ring wraps onto real cells
     # and its cells beyond the box are not stored
-    tx, ty = {0: (64, 8), 1: (32, 16), 2: (32, 8), 3: (32, 8)}[tile]
+    tx, ty = {0: (64, 8), 1: (32, 16), 2: (32, 8), 3: (32, 8), 4: (64, 16)}[tile]
     if shape[0] < tx or shape[1] < ty:
         pytest.skip("box narrower than the tile")
     d = _dom(*shape)
