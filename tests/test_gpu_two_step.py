@@ -304,6 +304,26 @@ def test_batched_two_step_equals_instances_alone(prec):
             assert np.array_equal(got[k], b.get_f())
 
 
+@pytest.mark.parametrize("prec", ["fp16", "bf16"])
+def test_batched_64x16_two_step_equals_instances_alone(prec):
+    # a batch whose instances are all at least 64 x 16 across, in the 64x16
+    # column tile with the partial prefetch: ragged last columns (100, 80)
+    # and sweeps cut at each instance's own depth
+    doms = [_dom(100, ny=40, nz=12, re=300), _dom(64, ny=64, nz=9, re=900, rho0=1.1),
+            _dom(80, ny=16, nz=5, re=150, u0=0.03)]
+    with _ctx(doms, precision=prec, two_step=1, two_step_tile=4, two_step_zseg=4, two_step_prefetch=2) as a:
+        a.initialize()
+        a.step(9)
+        info = a.info()
+        assert info["two_step"] == 1 and info["two_step_tile"] == 4 and info["two_step_prefetch"] == 2
+        got = [a.get_f(k) for k in range(3)]
+    for k, d in enumerate(doms):
+        with _ctx(d, precision=prec, two_step=0) as b:
+            b.initialize()
+            b.step(9)
+            assert np.array_equal(got[k], b.get_f()), k
+
+
 RAGGED = [("fp64", 3), ("fp64", 2), ("fp32", 0), ("fp32", 1), ("fp16", 0), ("fp16", 3), ("bf16", 1),
           ("fp16", 4), ("bf16", 4)]
 
