@@ -134,13 +134,14 @@ struct imex_ctx {
     std::vector<WorkTile> tiles;
     std::vector<HaloSeg> segs;
     std::vector<int2> pack_blocks, unpack_blocks, direct_blocks;
+    std::vector<int2> direct_blocks_nox;   // the direct fill without the x faces
     int rgrid[3] = {1, 1, 1};
 
     BoxDesc *d_boxes = nullptr;
     WorkTile *d_tiles = nullptr;
     size_t d_tiles_cap = 0;
     HaloSeg *d_segs = nullptr;
-    int2 *d_pack = nullptr, *d_unpack = nullptr, *d_direct = nullptr;
+    int2 *d_pack = nullptr, *d_unpack = nullptr, *d_direct = nullptr, *d_direct_nox = nullptr;
     double *d_tabs = nullptr;
     double *d_partial = nullptr;
     double *d_out = nullptr;
@@ -157,6 +158,8 @@ struct imex_ctx {
     bool state_S = true;       // pool[cur] holds S (true) or P with a pull owed
     bool pending = false;      // a Collision() not yet applied
     bool ghosts_valid = false; // tiled: ghost layers of pool[cur] are current
+    bool xghost_ok = false;    // tiled: the x ghost columns of pool[cur] were written by
+                               // the two-step pass that produced it (BoxDesc::xnb)
 
     int block = 256, cpt = 1, persistent = 0, nt = 0, dir_pad = 0, halo_depth = 1, xcds = 8;
     int contiguous = 0;
@@ -167,6 +170,8 @@ struct imex_ctx {
     bool halo_all = false;       // tiled two-step: ghosts carry all 27 directions
     bool halo_direct = true;     // tiled: one direct copy instead of pack + unpack
     long long direct_bytes = 0;  // bytes the direct fill moves
+    long long direct_bytes_nox = 0;
+    bool xdirect = false;        // the tiled two-step pass writes its x neighbours' ghosts
     PassBox *d_pass = nullptr;   // per-box workgroup table of the two-step pass
     int pass_key[2] = {-1, -1};  // tile and z-sweep the table was built for
     long long pass_grid = 0;
@@ -245,6 +250,7 @@ int layout_boxes(imex_ctx *c)
             // rank order me = px + rx*pz + rx*rz*py (src/mpi_view_transfer.cpp:96-98)
             const int p[3] = {me % r[0], me / r[0] / r[2], (me / r[0]) % r[2]};
             BoxDesc b{};
+            b.xnb[0] = b.xnb[1] = -1;
             int n[3];
             for (int a = 0; a < 3; ++a) {
                 n[a] = block_len(gl[a], r[a], p[a]);
@@ -383,7 +389,9 @@ void build_halo(imex_ctx *c)
     c->pack_blocks.clear();
     c->unpack_blocks.clear();
     c->direct_blocks.clear();
+    c->direct_blocks_nox.clear();
     std::vector<HaloSeg> direct;
+    std::vector<char> xface;   // direct segment of an x face (channel (+-1, 0, 0))
     std::vector<std::array<int, 5>> direct_key;   // box, channel (x, y, z), layer combination
     std::map<std::array<int, 5>, int> direct_at;
     const int d = c->halo_depth;
@@ -461,6 +469,7 @@ void build_halo(imex_ctx *c)
                             direct_at[{bi, cx, cy, cz, cb}] = (int)direct.size();
                             direct_key.push_back({I.box_begin + me, -cx, -cy, -cz, cb});
                             direct.push_back(ds);
+                            xface.push_back(cx != 0 && cy == 0 && cz == 0);
                         }
                     }
         }
@@ -481,16 +490,49 @@ void build_halo(imex_ctx *c)
                           direct[q].ez == direct[s].ez;
         if (same && q > s) direct[s].partner = nsu + q;
         if (same && q < s) continue;               // copied with its partner
-        for (int c0 = 0; c0 < direct[s].ncell; c0 += 256) c->direct_blocks.push_back({nsu + s, c0});
+        for (int c0 = 0; c0 < direct[s].ncell; c0 += 256) {
+            c->direct_blocks.push_back({nsu + s, c0});
+            if (!xface[s]) c->direct_blocks_nox.push_back({nsu + s, c0});
+        }
     }
     c->segs.insert(c->segs.end(), direct.begin(), direct.end());
     c->xbuf_elems = xoff;
     c->halo_bytes = 0;
     c->direct_bytes = 0;
-    for (const HaloSeg &sg : c->segs) {   // each segment reads and writes its cells
+    c->direct_bytes_nox = 0;
+    for (int s = 0; s < (int)c->segs.size(); ++s) {   // each segment reads and writes its cells
+        const HaloSeg &sg = c->segs[s];
         const long long b = 2LL * __builtin_popcount(sg.mask) * sg.ncell * c->ssize;
-        if (sg.pack == 2) c->direct_bytes += b;
-        else c->halo_bytes += b;
+        if (sg.pack == 2) {
+            c->direct_bytes += b;
+            if (!xface[s - nsu]) c->direct_bytes_nox += b;
+        } else {
+            c->halo_bytes += b;
+        }
+    }
+    // Two steps per pass with the direct fill: the pass writes the two x ghost
+    // columns of its x neighbours itself (every direction, from the values it
+    // stores), and the fill before the next pass leaves the x faces out.  The
+    // x neighbours of a box share its y and z extents (one Cartesian row).
+    for (BoxDesc &b : c->boxes) b.xnb[0] = b.xnb[1] = -1;
+    c->xdirect = c->halo_direct && c->halo_all && c->halo_depth == 2 && !c->whole_x && c->inst.size() == 1;
+    if (c->xdirect) {
+        const Instance &I = c->inst[0];
+        const int nb = I.box_end - I.box_begin;
+        for (int bi = I.box_begin; bi < I.box_end && c->xdirect; ++bi) {
+            const int *p = &c->box_coord[3 * bi];
+            for (int side = 0; side < 2; ++side) {
+                const int qx = (p[0] + (side ? 1 : -1) + c->rgrid[0]) % c->rgrid[0];
+                const int me = qx + c->rgrid[0] * p[2] + c->rgrid[0] * c->rgrid[2] * p[1];
+                const BoxDesc &b = c->boxes[bi];
+                if (me >= nb) { c->xdirect = false; break; }
+                const BoxDesc &t = c->boxes[I.box_begin + me];
+                if (t.ny != b.ny || t.nz != b.nz || t.nx < 2 || b.nx < 2) { c->xdirect = false; break; }
+                c->boxes[bi].xnb[side] = I.box_begin + me;
+            }
+        }
+        if (!c->xdirect)
+            for (BoxDesc &b : c->boxes) b.xnb[0] = b.xnb[1] = -1;
     }
 }
 
@@ -788,8 +830,12 @@ int launch_halo_t(imex_ctx *c, int which)
     S *f = (S *)pool_ptr(c, which);
     S *x = (S *)c->xbuf;
     if (c->halo_direct) {
-        const int nd = (int)c->direct_blocks.size();
-        if (nd) hipLaunchKernelGGL((k_halo<S>), dim3(nd), dim3(256), 0, c->stream, f, f, c->d_segs, c->d_direct, nd);
+        // x faces left out when the pass that produced the pool wrote them
+        const bool nox = c->xghost_ok && !c->capturing;
+        const int nd = (int)(nox ? c->direct_blocks_nox.size() : c->direct_blocks.size());
+        if (nd)
+            hipLaunchKernelGGL((k_halo<S>), dim3(nd), dim3(256), 0, c->stream, f, f, c->d_segs,
+                               nox ? c->d_direct_nox : c->d_direct, nd);
         HIPC(hipGetLastError());
         return IMEX_OK;
     }
@@ -803,7 +849,8 @@ int launch_halo_t(imex_ctx *c, int which)
 int ensure_ghosts(imex_ctx *c)
 {
     if (!c->tiled || c->ghosts_valid) return IMEX_OK;
-    PhaseScope ps(c, PH_HALO, (double)(c->halo_direct ? c->direct_bytes : c->halo_bytes));
+    PhaseScope ps(c, PH_HALO, (double)(c->halo_direct ? (c->xghost_ok ? c->direct_bytes_nox : c->direct_bytes)
+                                                       : c->halo_bytes));
     switch (c->prec) {
     case IMEX_F64: TRY(launch_halo_t<double>(c, c->cur)); break;
     case IMEX_F32: TRY(launch_halo_t<float>(c, c->cur)); break;
@@ -818,6 +865,7 @@ int do_collide(imex_ctx *c)
 {
     TRY(launch_step(c, OP_COLLIDE, c->cur, c->cur));
     c->ghosts_valid = false;
+    c->xghost_ok = false;
     return IMEX_OK;
 }
 
@@ -827,6 +875,7 @@ int do_fused(imex_ctx *c)
     TRY(launch_step(c, OP_FUSED, c->cur, 1 - c->cur));
     c->cur = 1 - c->cur;
     c->ghosts_valid = false;
+    c->xghost_ok = false;
     return IMEX_OK;
 }
 
@@ -836,6 +885,7 @@ int do_fused2(imex_ctx *c)
     TRY(launch_step2(c, c->cur, 1 - c->cur));
     c->cur = 1 - c->cur;
     c->ghosts_valid = false;
+    c->xghost_ok = c->xdirect && !c->capturing;
     return IMEX_OK;
 }
 
@@ -863,6 +913,7 @@ int do_stream(imex_ctx *c)
     TRY(launch_step(c, OP_STREAM, c->cur, 1 - c->cur));
     c->cur = 1 - c->cur;
     c->ghosts_valid = false;
+    c->xghost_ok = false;
     return IMEX_OK;
 }
 
@@ -905,6 +956,7 @@ int build_graph(imex_ctx *c, int parity)
     const bool save_gv = c->ghosts_valid;
     c->cur = parity;
     c->ghosts_valid = false;
+    c->xghost_ok = false;
     if (c->two_step) TRY(ts_prepare(c));
     HIPC(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     c->capturing = true;
@@ -935,6 +987,7 @@ int run_steps(imex_ctx *c, int nsteps)
                 HIPC(hipGraphLaunch(c->gexec[c->cur], c->stream));
                 if (c->gflip[c->cur]) c->cur = 1 - c->cur;
                 c->ghosts_valid = false;
+    c->xghost_ok = false;
             } else if (c->two_step) {
                 TRY(do_fused2(c));
             } else {
@@ -1114,7 +1167,8 @@ void imex_destroy(imex_ctx *c)
     drop_graphs(c);
     for (void *p : {(void *)c->d_boxes, (void *)c->d_tiles, (void *)c->d_segs, (void *)c->d_pack,
                     (void *)c->d_unpack, (void *)c->d_tabs, (void *)c->d_partial, (void *)c->d_out,
-                    (void *)c->d_stage, (void *)c->d_pass, (void *)c->d_direct, c->pool[0], c->pool[1], c->xbuf})
+                    (void *)c->d_stage, (void *)c->d_pass, (void *)c->d_direct, (void *)c->d_direct_nox, c->pool[0],
+                    c->pool[1], c->xbuf})
         if (p) (void)hipFree(p);
     if (c->h_out) (void)hipHostFree(c->h_out);
     for (hipEvent_t e : c->ev)
@@ -1282,12 +1336,15 @@ int imex_create(imex_ctx **out, const imex_config *cfg_in, const imex_domain *do
         if (hipMalloc(&c->d_segs, sb) != hipSuccess ||
             hipMalloc(&c->d_pack, std::max<size_t>(c->pack_blocks.size(), 1) * sizeof(int2)) != hipSuccess ||
             hipMalloc(&c->d_unpack, std::max<size_t>(c->unpack_blocks.size(), 1) * sizeof(int2)) != hipSuccess ||
-            hipMalloc(&c->d_direct, std::max<size_t>(c->direct_blocks.size(), 1) * sizeof(int2)) != hipSuccess)
+            hipMalloc(&c->d_direct, std::max<size_t>(c->direct_blocks.size(), 1) * sizeof(int2)) != hipSuccess ||
+            hipMalloc(&c->d_direct_nox, std::max<size_t>(c->direct_blocks_nox.size(), 1) * sizeof(int2)) != hipSuccess)
             return bail(fail(IMEX_ENOMEM, "halo table allocation failed"));
         if (hipMemcpy(c->d_segs, c->segs.data(), sb, hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(c->d_pack, c->pack_blocks.data(), c->pack_blocks.size() * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(c->d_unpack, c->unpack_blocks.data(), c->unpack_blocks.size() * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(c->d_direct, c->direct_blocks.data(), c->direct_blocks.size() * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess)
+            hipMemcpy(c->d_direct, c->direct_blocks.data(), c->direct_blocks.size() * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(c->d_direct_nox, c->direct_blocks_nox.data(), c->direct_blocks_nox.size() * sizeof(int2),
+                      hipMemcpyHostToDevice) != hipSuccess)
             return bail(fail(IMEX_EHIP, "halo table upload failed"));
     }
     const size_t bb = c->boxes.size() * sizeof(BoxDesc);
@@ -1333,6 +1390,7 @@ int imex_initialize(imex_ctx *c)
     c->pending = false;
     c->deferred = 0;
     c->ghosts_valid = false;
+    c->xghost_ok = false;
     HIPC(hipStreamSynchronize(c->stream));
     return IMEX_OK;
 }
@@ -1498,6 +1556,7 @@ int imex_set_f(imex_ctx *c, int domain, const double *host)
         HIPC(hipStreamSynchronize(c->stream));
     }
     c->ghosts_valid = false;
+    c->xghost_ok = false;
     return IMEX_OK;
 }
 
@@ -1634,7 +1693,9 @@ int imex_autotune(imex_ctx *c, int reps, double *best_ms)
 
     // Two steps per pass: column tile x z-planes per sweep x tile order,
     // timed per pass (two steps) against two single-step launches.
-    if (c->cfg.two_step != 0) {
+    // (a tiled context only when its ghost tables were built for two steps per
+    // pass: two layers, every direction of the first)
+    if (c->cfg.two_step != 0 && (!c->tiled || c->halo_all)) {
         struct Cand2 { int tile, zseg, xcds, pf, buf; };
         std::vector<Cand2> c2;
         const int want_pf = c->cfg.two_step_prefetch;

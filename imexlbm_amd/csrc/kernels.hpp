@@ -41,6 +41,9 @@ struct BoxDesc {
     long long base;     // element offset of the box allocation (ghosts included)
     long long zorg;     // offset of interior z-plane 0 from base (3*sz with ghosts)
     int wrap_x;         // box with ghosts in y and z only: x wraps in the kernels
+    int xnb[2];         // tiled two-step pass: boxes whose two x ghost columns this
+                        // box's first / last two columns fill (the lower / upper
+                        // x neighbour), written by the pass itself; -1: none
 };
 
 struct WorkTile { int box, z, p0, n; };
@@ -517,6 +520,18 @@ __device__ inline void load_cell_head_raw(const PL &P, int dsb, int xy, const Nb
     ((h[I] = P.template raw<EZ[I]>((xy + pull_xy<I>(n)) * 2, I * dsb)), ...);
 }
 
+// One cell's populations into x ghost column x (-2, -1, nx or nx + 1) of box
+// t, row y and z-plane z (interior rows and planes).
+template <class S, bool NT, bool GIO>
+__device__ inline void x_ghost_store(S *dst, const BoxDesc &t, int x, int y, int z,
+                                     const typename Store<S>::real (&f)[Q])
+{
+    S *wp = dst + t.base + t.zorg + (long long)z * t.sz;
+    const Planes<S, GIO> P(wp, wp, wp, wp);
+    store_cell_io<S, NT>(P, (int)(t.dstride * (long long)sizeof(S)), t.org_xy + x + y * t.sy, f,
+                         std::make_integer_sequence<int, Q>{});
+}
+
 // Directions of the partial prefetch of 2-byte formats (PD): the first 11
 // populations of each thread's first ring cell.  More made no further gain at
 // 512^3; every thread owns a ring cell in the tiles that use it.
@@ -599,6 +614,16 @@ k_step2(const S *__restrict__ src, S *__restrict__ dst, const BoxDesc *__restric
             if constexpr (Store<S>::shifted) collide32(f, d.omega_f, collide_one<S, WRAP>());
             else collide64(f, d.omega);
             store_cell_io<S, NT>(P, dsb, d.org_xy + (x0 + tx) + (y0 + ty) * d.sy, f, seq);
+            if constexpr (!WRAP) {
+                // the first and last two columns are also the x neighbours'
+                // two ghost columns: stored there too, every direction
+                const int gx = x0 + tx, gy = y0 + ty;
+                if (gx < 2 && d.xnb[0] >= 0) {
+                    const BoxDesc &t = boxes[d.xnb[0]];
+                    x_ghost_store<S, NT, GIO>(dst, t, t.nx + gx, gy, k, f);
+                }
+                if (gx >= d.nx - 2 && d.xnb[1] >= 0) x_ghost_store<S, NT, GIO>(dst, boxes[d.xnb[1]], gx - d.nx, gy, k, f);
+            }
         }
     };
     first(z0 - 1);
