@@ -179,6 +179,23 @@ def test_tiled_autotune_keeps_state(prec, whole_x):
         assert np.array_equal(a.get_f(), b.get_f())
 
 
+def test_tiled_autotune_keeps_one_step_ghost_tables():
+    # two tiles of 64 x 64 x 32: too few workgroups for the model to pick two
+    # steps per pass, so the ghost tables are built one layer deep; the
+    # autotuner must not switch to two steps on them
+    from imexlbm_amd import capi
+    d = _dom(64)
+    with _ctx(d, precision="fp32", mode=capi.TILED, ntiles=2, halo_depth=1) as a, \
+            _ctx(d, precision="fp32", two_step=0) as b:
+        assert a.info()["two_step"] == 0
+        a.initialize(); b.initialize()
+        a.step(3); b.step(3)
+        assert a.autotune(reps=2) > 0
+        assert a.info()["two_step"] == 0
+        a.step(8); b.step(8)
+        assert np.array_equal(a.get_f(), b.get_f())
+
+
 def test_two_step_eligibility():
     from imexlbm_amd import capi
     with _ctx(_dom(32), precision="fp64") as c:   # auto: 4 column tiles are too few workgroups
