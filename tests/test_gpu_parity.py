@@ -334,10 +334,12 @@ def test_tiled_256_three_layer_halo_matches_wrap(direct, two, whole_x):
     assert abs(cw[0] - ct[0]) / cw[0] <= 1e-13 and abs(cw[4] - ct[4]) / cw[4] <= 1e-13
 
 
+@pytest.mark.parametrize("whole_x", [1, 0])
 @pytest.mark.parametrize("prec", ["fp16", "bf16"])
-def test_tiled_256_16bit_whole_x_matches_wrap(prec):
-    # the tiled benchmark's shape: 1x2x4 tiles keeping x whole, two steps
-    # per pass in 64x16 columns (FP16 with the partial prefetch and its
+def test_tiled_256_16bit_whole_x_matches_wrap(prec, whole_x):
+    # the tiled benchmark's shapes: 1x2x4 tiles keeping x whole, or the
+    # reference's 2x2x2 with the pass writing its x neighbours' ghosts; two
+    # steps per pass in 64x16 columns (FP16 with the partial prefetch and its
     # mixed-precision collision), against the periodic box one step per
     # pass, bit for bit
     from imexlbm_amd import capi
@@ -347,12 +349,12 @@ def test_tiled_256_16bit_whole_x_matches_wrap(prec):
         w.initialize(); w.step(6)
         cw = w.conserved()
         fw = w.get_f()
-    with _ctx(d, precision=prec, mode=capi.TILED, ntiles=8, halo_depth=3, whole_x=1, two_step=1,
+    with _ctx(d, precision=prec, mode=capi.TILED, ntiles=8, halo_depth=3, whole_x=whole_x, two_step=1,
               two_step_tile=4, two_step_prefetch=pf) as t:
         t.initialize(); t.step(6)
         ct = t.conserved()
         info = t.info()
-        assert (info["rx"], info["ry"], info["rz"]) == (1, 2, 4)
+        assert (info["rx"], info["ry"], info["rz"]) == ((1, 2, 4) if whole_x else (2, 2, 2))
         assert info["two_step_tile"] == 4 and info["two_step_prefetch"] == pf
         assert np.array_equal(fw, t.get_f())
     del fw
