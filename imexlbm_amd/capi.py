@@ -49,7 +49,8 @@ class Info(ctypes.Structure):
                     "cells_per_thread", "persistent", "nontemporal", "dir_pad", "use_graph",
                     "grid", "rx", "ry", "rz", "xcd_groups", "contiguous", "io16", "xcd_run",
                     "two_step", "two_step_tile", "two_step_zseg", "two_step_xcds", "two_step_prefetch",
-                    "two_step_io", "halo_direct", "halo_depth", "whole_x")]
+                    "two_step_io", "halo_direct", "halo_depth", "whole_x")] + [
+                        ("halo_send_bytes", ctypes.c_longlong)]
 
 
 class ImexError(RuntimeError):
@@ -220,7 +221,7 @@ class Context:
         return float(b[0])
 
     def profile(self):
-        """{phase name: (calls, total ms, fastest ms, slowest ms, bytes)}"""
+        """{group name: (calls, total ms, fastest ms, slowest ms, bytes, flops)}"""
         L = load()
         out = {}
         k = 0
@@ -228,7 +229,7 @@ class Context:
             name = L.imex_profile_phase_name(k)
             if name is None:
                 return out
-            v = np.zeros(4, np.float64)
+            v = np.zeros(5, np.float64)
             n = ctypes.c_longlong()
             _check(L.imex_profile_read(self._h, k, _dptr(v), ctypes.byref(n)))
             out[name.decode()] = (n.value, *map(float, v))

@@ -104,14 +104,25 @@ void cartesian(int n, const int gl[3], int r[3], bool whole_x = false)
 int block_len(int gl, int r, int p) { return (p - gl % r >= 0) ? gl / r : gl / r + 1; }
 int block_lo(int gl, int r, int p) { return p * (gl / r) + std::min(p, gl % r); }
 
-// Phases of the per-phase profiler (the counterpart of the reference's
-// LBM_PROFILE build, src/profile.hpp:42-113).
-enum Phase { PH_COLLIDE = 0, PH_FUSED, PH_STREAM, PH_HALO, PH_CONSERVED, PH_MACRO, PH_FUSED2, PH_N };
-const char *const kPhaseNames[PH_N] = {"collide in place (first step)", "fused pull-stream + collide",
-                                       "bare stream", "ghost fill (pack + unpack)", "conserved sums",
-                                       "macroscopic fields", "two fused steps in one pass"};
+// Groups of the per-phase profiler, as the reference's LBM_PROFILE build
+// reports them (src/profile.cpp:64-97): the first six carry its group names,
+// byte model (collision and stream 2 x cells x 27 x sizeof(storage), pack and
+// unpack twice the exchange-buffer bytes, computeMacroscopic the populations
+// plus four doubles per cell) and flop model (594 per collided cell, 189 per
+// cell of moments).  On one GPU the pack buffer is read by the unpack
+// directly, so "mpi" records nothing.  The last three have no counterpart
+// there: the fused pull-stream + collide launch (one or two lattice updates;
+// its bytes are those the launch must move, every population read once and
+// written once), the direct ghost fill and the conserved sums.
+enum Phase { PH_COLLISION = 0, PH_PACK, PH_MPI, PH_UNPACK, PH_STREAM, PH_MACRO, PH_FUSED, PH_DIRECT,
+             PH_CONSERVED, PH_N };
+const char *const kPhaseNames[PH_N] = {"collision", "pack (enq+fence)", "mpi (post+wait)",
+                                       "unpack (enq+fence)", "stream (enq+fence)", "computeMacroscopic",
+                                       "collision+stream (fused)", "ghost fill (direct)", "conserved"};
+constexpr double FLOP_COLLISION = 594.0;   // src/profile.cpp:38, src/main.cpp:49
+constexpr double FLOP_MOMENTS = 189.0;     // src/profile.cpp:39
 struct PhaseStat {
-    double ms = 0, mn = 0, mx = 0, bytes = 0;
+    double ms = 0, mn = 0, mx = 0, bytes = 0, flops = 0;
     long long calls = 0;
 };
 
@@ -183,6 +194,7 @@ struct imex_ctx {
     long long cells = 0;
     long long device_bytes = 0;
     long long halo_bytes = 0;    // bytes one pack + unpack moves
+    long long pack_bytes = 0;    // bytes the pack launch moves (2 x the buffer)
 
     bool profile = false;        // time every phase with events (adds a sync)
     bool capturing = false;      // inside a hipGraph capture: no timing
@@ -197,9 +209,10 @@ namespace {
 struct PhaseScope {
     imex_ctx *c;
     int p;
-    double bytes;
+    double bytes, flops;
     bool on;
-    PhaseScope(imex_ctx *c_, int p_, double b) : c(c_), p(p_), bytes(b), on(c_->profile && !c_->capturing)
+    PhaseScope(imex_ctx *c_, int p_, double b, double fl = 0.0)
+        : c(c_), p(p_), bytes(b), flops(fl), on(c_->profile && !c_->capturing)
     {
         if (on) (void)hipEventRecord(c->pev[0], c->stream);
     }
@@ -216,6 +229,7 @@ struct PhaseScope {
         st.mx = st.calls ? std::max(st.mx, (double)ms) : ms;
         st.ms += ms;
         st.bytes += bytes;
+        st.flops += flops;
         ++st.calls;
     }
 };
@@ -498,6 +512,7 @@ void build_halo(imex_ctx *c)
     c->segs.insert(c->segs.end(), direct.begin(), direct.end());
     c->xbuf_elems = xoff;
     c->halo_bytes = 0;
+    c->pack_bytes = 0;
     c->direct_bytes = 0;
     c->direct_bytes_nox = 0;
     for (int s = 0; s < (int)c->segs.size(); ++s) {   // each segment reads and writes its cells
@@ -508,6 +523,7 @@ void build_halo(imex_ctx *c)
             if (!xface[s - nsu]) c->direct_bytes_nox += b;
         } else {
             c->halo_bytes += b;
+            if (sg.pack == 1) c->pack_bytes += b;
         }
     }
     // Two steps per pass with the direct fill: the pass writes the two x ghost
@@ -579,8 +595,8 @@ int launch_step_t(imex_ctx *c, int op, int src, int dst, int t0, int n)
 
 int launch_step(imex_ctx *c, int op, int src, int dst)
 {
-    PhaseScope ps(c, op == OP_FUSED ? PH_FUSED : (op == OP_COLLIDE ? PH_COLLIDE : PH_STREAM),
-                  2.0 * Q * c->ssize * (double)c->cells);
+    PhaseScope ps(c, op == OP_FUSED ? PH_FUSED : (op == OP_COLLIDE ? PH_COLLISION : PH_STREAM),
+                  2.0 * Q * c->ssize * (double)c->cells, op == OP_STREAM ? 0.0 : FLOP_COLLISION * (double)c->cells);
     auto one = [&](int t0, int n) -> int {
         if (n <= 0) return IMEX_OK;
         switch (c->prec) {
@@ -815,7 +831,7 @@ int launch_step2_t(imex_ctx *c, int src, int dst)
 // ts_allowed for c->ts_tile).
 int launch_step2(imex_ctx *c, int src, int dst)
 {
-    PhaseScope ps(c, PH_FUSED2, 2.0 * 2.0 * Q * c->ssize * (double)c->cells);
+    PhaseScope ps(c, PH_FUSED, 2.0 * Q * c->ssize * (double)c->cells, 2.0 * FLOP_COLLISION * (double)c->cells);
     switch (c->prec) {
     case IMEX_F64: return launch_step2_t<double>(c, src, dst);
     case IMEX_F32: return launch_step2_t<float>(c, src, dst);
@@ -833,6 +849,7 @@ int launch_halo_t(imex_ctx *c, int which)
         // x faces left out when the pass that produced the pool wrote them
         const bool nox = c->xghost_ok && !c->capturing;
         const int nd = (int)(nox ? c->direct_blocks_nox.size() : c->direct_blocks.size());
+        PhaseScope ps(c, PH_DIRECT, (double)(nox ? c->direct_bytes_nox : c->direct_bytes));
         if (nd)
             hipLaunchKernelGGL((k_halo<S>), dim3(nd), dim3(256), 0, c->stream, f, f, c->d_segs,
                                nox ? c->d_direct_nox : c->d_direct, nd);
@@ -840,7 +857,12 @@ int launch_halo_t(imex_ctx *c, int which)
         return IMEX_OK;
     }
     const int np = (int)c->pack_blocks.size(), nu = (int)c->unpack_blocks.size();
-    if (np) hipLaunchKernelGGL((k_halo<S>), dim3(np), dim3(256), 0, c->stream, f, x, c->d_segs, c->d_pack, np);
+    {
+        PhaseScope ps(c, PH_PACK, (double)c->pack_bytes);
+        if (np) hipLaunchKernelGGL((k_halo<S>), dim3(np), dim3(256), 0, c->stream, f, x, c->d_segs, c->d_pack, np);
+        HIPC(hipGetLastError());
+    }
+    PhaseScope ps(c, PH_UNPACK, (double)(c->halo_bytes - c->pack_bytes));
     if (nu) hipLaunchKernelGGL((k_halo<S>), dim3(nu), dim3(256), 0, c->stream, x, f, c->d_segs, c->d_unpack, nu);
     HIPC(hipGetLastError());
     return IMEX_OK;
@@ -849,8 +871,6 @@ int launch_halo_t(imex_ctx *c, int which)
 int ensure_ghosts(imex_ctx *c)
 {
     if (!c->tiled || c->ghosts_valid) return IMEX_OK;
-    PhaseScope ps(c, PH_HALO, (double)(c->halo_direct ? (c->xghost_ok ? c->direct_bytes_nox : c->direct_bytes)
-                                                       : c->halo_bytes));
     switch (c->prec) {
     case IMEX_F64: TRY(launch_halo_t<double>(c, c->cur)); break;
     case IMEX_F32: TRY(launch_halo_t<float>(c, c->cur)); break;
@@ -1039,18 +1059,18 @@ int launch_export_t(imex_ctx *c, int box, bool gather)
 }
 
 template <class S>
-int launch_macro_t(imex_ctx *c, int box, bool gather)
+int launch_macro_t(imex_ctx *c, int box, bool gather, double *out)
 {
     const S *sp = (const S *)pool_ptr(c, c->cur);
     const BoxDesc &b = c->boxes[box];
     const long long ncell = (long long)b.nx * b.ny * b.nz;
     const int grid = (int)std::min<long long>((ncell + 255) / 256, 65536);
     if (!c->tiled) {
-        if (gather) hipLaunchKernelGGL((k_macro<S, true, true>), dim3(grid), dim3(256), 0, c->stream, sp, c->d_boxes, box, c->d_stage);
-        else hipLaunchKernelGGL((k_macro<S, true, false>), dim3(grid), dim3(256), 0, c->stream, sp, c->d_boxes, box, c->d_stage);
+        if (gather) hipLaunchKernelGGL((k_macro<S, true, true>), dim3(grid), dim3(256), 0, c->stream, sp, c->d_boxes, box, out);
+        else hipLaunchKernelGGL((k_macro<S, true, false>), dim3(grid), dim3(256), 0, c->stream, sp, c->d_boxes, box, out);
     } else {
-        if (gather) hipLaunchKernelGGL((k_macro<S, false, true>), dim3(grid), dim3(256), 0, c->stream, sp, c->d_boxes, box, c->d_stage);
-        else hipLaunchKernelGGL((k_macro<S, false, false>), dim3(grid), dim3(256), 0, c->stream, sp, c->d_boxes, box, c->d_stage);
+        if (gather) hipLaunchKernelGGL((k_macro<S, false, true>), dim3(grid), dim3(256), 0, c->stream, sp, c->d_boxes, box, out);
+        else hipLaunchKernelGGL((k_macro<S, false, false>), dim3(grid), dim3(256), 0, c->stream, sp, c->d_boxes, box, out);
     }
     HIPC(hipGetLastError());
     return IMEX_OK;
@@ -1447,7 +1467,8 @@ int imex_conserved(imex_ctx *c, int domain, double out[5])
     const bool gather = !c->state_S;
     if (gather) TRY(ensure_ghosts(c));
     const Instance &I = c->inst[domain];
-    PhaseScope ps(c, PH_CONSERVED, (double)Q * c->ssize * I.dom.nx * (double)I.dom.ny * I.dom.nz);
+    PhaseScope ps(c, PH_CONSERVED, (double)Q * c->ssize * I.dom.nx * (double)I.dom.ny * I.dom.nz,
+                  FLOP_MOMENTS * I.dom.nx * (double)I.dom.ny * I.dom.nz);
     TRY(DISPATCH(launch_conserved_t, c, I.tile_begin, I.tile_end - I.tile_begin, gather));
     HIPC(hipMemcpyAsync(c->h_out, c->d_out, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPC(hipStreamSynchronize(c->stream));
@@ -1498,25 +1519,37 @@ int imex_get_macro(imex_ctx *c, int domain, double *p, double *u, double *v, dou
     const Instance &I = c->inst[domain];
     const long long NY = I.dom.ny, NZ = I.dom.nz;
     double *outs[4] = {p, u, v, w};
+    // every box's fields into one staging area, so that the macroscopic
+    // kernels of one call are timed as one phase
+    std::vector<size_t> at;
+    size_t total = 0;
+    for (int bi = I.box_begin; bi < I.box_end; ++bi) {
+        const BoxDesc &b = c->boxes[bi];
+        at.push_back(total);
+        total += 4 * (size_t)b.nx * b.ny * b.nz;
+    }
+    TRY(ensure_stage(c, total));
+    {
+        const double ncell = (double)I.dom.nx * I.dom.ny * I.dom.nz;
+        PhaseScope ps(c, PH_MACRO, (Q * c->ssize + 32.0) * ncell, FLOP_MOMENTS * ncell);
+        for (int bi = I.box_begin; bi < I.box_end; ++bi)
+            TRY(DISPATCH(launch_macro_t, c, bi, gather, c->d_stage + at[bi - I.box_begin]));
+    }
     std::vector<double> tmp;
     for (int bi = I.box_begin; bi < I.box_end; ++bi) {
         const BoxDesc &b = c->boxes[bi];
         const long long ncell = (long long)b.nx * b.ny * b.nz;
-        TRY(ensure_stage(c, 4 * (size_t)ncell));
-        {
-            PhaseScope ps(c, PH_MACRO, (Q * c->ssize + 32.0) * (double)ncell);
-            TRY(DISPATCH(launch_macro_t, c, bi, gather));
-        }
         tmp.resize(4 * (size_t)ncell);
-        HIPC(hipMemcpyAsync(tmp.data(), c->d_stage, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPC(hipMemcpyAsync(tmp.data(), c->d_stage + at[bi - I.box_begin], tmp.size() * sizeof(double),
+                            hipMemcpyDeviceToHost, c->stream));
         HIPC(hipStreamSynchronize(c->stream));
-        for (int q = 0; q < 4; ++q)
+        for (int m = 0; m < 4; ++m)
             for (int k = 0; k < b.nz; ++k)
                 for (int j = 0; j < b.ny; ++j)
                     for (int i = 0; i < b.nx; ++i) {
                         const long long X = b.lo[0] + i, Y = b.lo[1] + j, Z = b.lo[2] + k;
-                        outs[q][(X * NY + Y) * NZ + Z] =
-                            tmp[(size_t)q * ncell + ((size_t)k * b.ny + j) * b.nx + i];
+                        outs[m][(X * NY + Y) * NZ + Z] =
+                            tmp[(size_t)m * ncell + ((size_t)k * b.ny + j) * b.nx + i];
                     }
     }
     return IMEX_OK;
@@ -1563,7 +1596,7 @@ int imex_set_f(imex_ctx *c, int domain, const double *host)
 int imex_synchronize(imex_ctx *c)
 {
     TRY(check_ctx(c));
-    TRY(flush_deferred(c));
+    TRY(flush_pending(c));
     HIPC(hipStreamSynchronize(c->stream));
     return IMEX_OK;
 }
@@ -1835,7 +1868,7 @@ int imex_profile_reset(imex_ctx *c)
     return IMEX_OK;
 }
 
-int imex_profile_read(imex_ctx *c, int phase, double out[4], long long *calls)
+int imex_profile_read(imex_ctx *c, int phase, double out[5], long long *calls)
 {
     if (!c || !out || !calls) return fail(IMEX_EINVAL, "null argument");
     if (phase < 0 || phase >= PH_N) return fail(IMEX_EINVAL, "phase out of range");
@@ -1844,6 +1877,7 @@ int imex_profile_read(imex_ctx *c, int phase, double out[4], long long *calls)
     out[1] = p.mn;
     out[2] = p.mx;
     out[3] = p.bytes;
+    out[4] = p.flops;
     *calls = p.calls;
     return IMEX_OK;
 }
@@ -1877,6 +1911,7 @@ int imex_query(imex_ctx *c, imex_info *info)
     info->two_step_io = c->ts_buf ? 0 : 1;
     info->halo_direct = c->tiled ? (c->halo_direct ? 1 : 0) : 0;
     info->whole_x = c->whole_x ? 1 : 0;
+    info->halo_send_bytes = c->tiled ? c->pack_bytes / 2 : 0;
     info->halo_depth = c->tiled ? c->halo_depth : 0;
     info->grid = c->tiled ? 1 : 0;
     info->rx = c->rgrid[0]; info->ry = c->rgrid[1]; info->rz = c->rgrid[2];

@@ -3,10 +3,18 @@
 // reference.
 #include "lbm.hpp"
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
+
+// Peak device memory bandwidth per GPU for the profiler's "% BW" column;
+// the reference assumes 1600 GB/s (src/profile.hpp:67-73), MI355X's
+// published HBM3E figure is 8 TB/s.  Override with -DLBM_PROF_PEAK_GBPS=...
+#ifndef LBM_PROF_PEAK_GBPS
+#define LBM_PROF_PEAK_GBPS 8000.0
+#endif
 
 namespace {
 
@@ -27,7 +35,7 @@ int env_int(const char *name, int dflt)
 
 }  // namespace
 
-LBM::LBM(lbm_comm comm_, int sx, int sy, int sz, double &tau, double &rho0_, double &u0_)
+LBM::LBM(MPI_Comm comm_, int sx, int sy, int sz, double &tau, double &rho0_, double &u0_)
     : comm(comm_), glx(sx), gly(sy), glz(sz), rho0(rho0_), tau0(tau), u0(u0_)
 {
     cs2 = 1.0 / 3.0;
@@ -51,9 +59,14 @@ LBM::LBM(lbm_comm comm_, int sx, int sy, int sz, double &tau, double &rho0_, dou
     }
     cfg.ntiles = env_int("IMEXLBM_TILES", 8);
     cfg.halo_depth = env_int("IMEXLBM_HALO", 1);
+#ifdef LBM_PROFILE
+    cfg.profile = env_int("IMEXLBM_PROFILE", 1);
+#else
     cfg.profile = env_int("IMEXLBM_PROFILE", 0);
+#endif
     cfg.two_step = env_int("IMEXLBM_TWO_STEP", -1);
     cfg.whole_x = env_int("IMEXLBM_WHOLE_X", 0);
+    cfg.halo_direct = env_int("IMEXLBM_HALO_DIRECT", -1);
 
     // one rank owns the whole domain (src/mpi_view_transfer.cpp:163-195 at rx=ry=rz=1)
     l_l[0] = glx; l_l[1] = gly; l_l[2] = glz;
@@ -69,6 +82,19 @@ LBM::~LBM()
 
 void LBM::Initialize()
 {
+    // setup_MPI (src/mpi_view_transfer.cpp:41-51): this process's rank and
+    // the communicator's size.  One GPU process owns the whole domain here.
+    MPI_Comm_rank(comm, &me);
+    MPI_Comm_size(comm, &nranks);
+    if (nranks != 1)
+    {
+        if (me == 0)
+            std::fprintf(stderr,
+                         "FATAL: this build advances the whole domain on one GPU per process; "
+                         "run it with one MPI rank (got %d)\n", nranks);
+        std::fflush(stderr);
+        MPI_Abort(comm, 1);
+    }
     imex_domain d;
     d.nx = glx; d.ny = gly; d.nz = glz;
     d.tau0 = tau0; d.rho0 = rho0; d.u0 = u0;
@@ -89,6 +115,8 @@ void LBM::Conserved(double &mass, double &mom_x, double &mom_y, double &mom_z, d
 {
     double out[5];
     check(imex_conserved(ctx, 0, out), "imex_conserved");
+    // the reference's global reduction of the five sums (src/lbm.cpp:297-299)
+    MPI_Allreduce(MPI_IN_PLACE, out, 5, MPI_DOUBLE, MPI_SUM, comm);
     mass = out[0]; mom_x = out[1]; mom_y = out[2]; mom_z = out[3]; ke = out[4];
 }
 
@@ -96,40 +124,105 @@ void LBM::fence() { check(imex_synchronize(ctx), "imex_synchronize"); }
 
 void LBM::download_f(double *host) { check(imex_get_f(ctx, 0, host), "imex_get_f"); }
 
-void LBM::ProfileReset()
+void LBM::ProfileStepBegin()
 {
-    if (cfg.profile) check(imex_profile_reset(ctx), "imex_profile_reset");
+    if (cfg.profile) step_t0 = MPI_Wtime();
 }
 
-// Table in the spirit of src/profile.cpp:99-204: per phase the mean time per
-// timed step, the fastest and slowest call, the share of the step, and the
-// algorithmic bandwidth against a copy ceiling measured on this device
-// (where the reference assumes 1600 GB/s).
+void LBM::ProfileStepEnd()
+{
+    if (!cfg.profile) return;
+    // every launch of a profiled context completes before its call returns
+    const double dt = MPI_Wtime() - step_t0;
+    step_min = step_calls ? std::min(step_min, dt) : dt;
+    step_max = step_calls ? std::max(step_max, dt) : dt;
+    step_acc += dt;
+    ++step_calls;
+}
+
+void LBM::ProfileReset()
+{
+    if (!cfg.profile) return;
+    check(imex_profile_reset(ctx), "imex_profile_reset");
+    step_acc = step_min = step_max = 0.0;
+    step_calls = 0;
+}
+
+// The reference's report (src/profile.cpp:101-204) for one GPU: the phase
+// table (per-call mean, fastest and slowest call, spread and share of the
+// step) and the group table, the only place rates are given: GFLOP/s from
+// 594 flop per collided cell and 189 per cell of moments, GB/s from the
+// algorithmic bytes of each group, and "% BW" against LBM_PROF_PEAK_GBPS.
 void LBM::ProfileReport(int nsteps)
 {
-    if (!cfg.profile || nsteps <= 0) return;
-    double peak = 0.0;
-    if (imex_copy_probe(cfg.device, 4LL << 30, 5, &peak) != IMEX_OK) peak = 0.0;
-    std::printf("\n=== per-phase profile: 1 GPU, %d timed steps ===\n", nsteps);
-    std::printf("%-34s %6s %10s %9s %9s %7s %9s %7s\n", "phase", "calls", "ms/step", "min ms", "max ms",
-                "step%", "GB/s", "%copy");
-    double step_ms = 0.0;
-    for (int p = 0; imex_profile_phase_name(p); ++p) {
-        double v[4];
-        long long n;
-        check(imex_profile_read(ctx, p, v, &n), "imex_profile_read");
-        step_ms += v[0];
+    if (!cfg.profile) return;
+    if (nsteps <= 0)
+    {
+        if (me == 0) std::printf("\n[profile] nothing to report (nsteps <= 0)\n");
+        return;
     }
-    for (int p = 0; imex_profile_phase_name(p); ++p) {
-        double v[4];
+    imex_info info;
+    check(imex_query(ctx, &info), "imex_query");
+    if (me != 0) return;
+
+    const double step = step_calls ? 1e3 * step_acc / step_calls : 0.0;
+    std::printf("\n=== per-phase profile: %d ranks, %d timed steps ===\n", nranks, nsteps);
+    std::printf("mean ms = per-call mean; spread%% = (slowest - fastest) / mean\n");
+    std::printf("imbal   = max/min across ranks (1.00 with one rank)\n");
+    std::printf("device groups are timed with HIP events, one synchronisation per launch\n\n");
+    std::printf("%-36s %6s %9s %8s %8s %7s %6s %7s\n", "phase", "calls", "mean ms", "min ms", "max ms",
+                "spread%", "imbal", "% step");
+    std::printf("%-36s %6s %9s %8s %8s %7s %6s %7s\n", "------------------------------------", "------",
+                "---------", "--------", "--------", "-------", "------", "-------");
+    std::printf("%-36s %6ld %9.4f %8.4f %8.4f %7.1f %6.2f %7.1f\n", "STEP TOTAL", step_calls, step,
+                1e3 * step_min, 1e3 * step_max, step > 0 ? 100.0 * 1e3 * (step_max - step_min) / step : 0.0,
+                1.0, 100.0);
+    double attributed = 0.0;   // per step
+    for (int p = 0; imex_profile_phase_name(p); ++p)
+    {
+        double v[5];
         long long n;
         check(imex_profile_read(ctx, p, v, &n), "imex_profile_read");
         if (n == 0) continue;
-        const double gbs = v[0] > 0 ? v[3] / (v[0] * 1e-3) / 1e9 : 0.0;
-        std::printf("%-34s %6lld %10.4f %9.4f %9.4f %7.1f %9.1f %7.1f\n", imex_profile_phase_name(p), n,
-                    v[0] / nsteps, v[1], v[2], step_ms > 0 ? 100.0 * v[0] / step_ms : 0.0, gbs,
-                    peak > 0 ? 100.0 * gbs / peak : 0.0);
+        const double mean = v[0] / n, per_step = v[0] / nsteps;
+        attributed += per_step;
+        std::printf("  %-34s %6lld %9.4f %8.4f %8.4f %7.1f %6.2f %7.1f\n", imex_profile_phase_name(p), n, mean,
+                    v[1], v[2], mean > 0 ? 100.0 * (v[2] - v[1]) / mean : 0.0, 1.0,
+                    step > 0 ? 100.0 * per_step / step : 0.0);
     }
-    std::printf("copy ceiling measured on this device: %.1f GB/s\n", peak);
-    std::printf("profile run (one synchronisation per phase): take throughput from the MLUPS lines\n\n");
+    std::printf("%-36s %6s %9.4f %8s %8s %7s %6s %7.1f\n", "  (unattributed)", "", step - attributed, "", "",
+                "", "", step > 0 ? 100.0 * (step - attributed) / step : 0.0);
+
+    std::printf("\n%-26s %9s %7s %10s %10s %7s\n", "group", "ms/step", "% step", "GFLOP/s", "GB/s", "% BW");
+    std::printf("%-26s %9s %7s %10s %10s %7s\n", "--------------------------", "---------", "-------",
+                "----------", "----------", "-------");
+    for (int p = 0; imex_profile_phase_name(p); ++p)
+    {
+        double v[5];
+        long long n;
+        check(imex_profile_read(ctx, p, v, &n), "imex_profile_read");
+        if (n == 0) continue;
+        const double secs = 1e-3 * v[0];
+        char gf[16], gb[16], bw[16];
+        if (v[4] > 0 && secs > 0) std::snprintf(gf, sizeof gf, "%10.1f", v[4] / secs / 1e9);
+        else std::snprintf(gf, sizeof gf, "%10s", "-");
+        if (v[3] > 0 && secs > 0)
+        {
+            const double gbs = v[3] / secs / 1e9;
+            std::snprintf(gb, sizeof gb, "%10.1f", gbs);
+            std::snprintf(bw, sizeof bw, "%7.1f", 100.0 * gbs / LBM_PROF_PEAK_GBPS);
+        }
+        else
+        {
+            std::snprintf(gb, sizeof gb, "%10s", "-");
+            std::snprintf(bw, sizeof bw, "%7s", "-");
+        }
+        std::printf("%-26s %9.4f %7.1f %s %s %s\n", imex_profile_phase_name(p), v[0] / nsteps,
+                    step > 0 ? 100.0 * v[0] / nsteps / step : 0.0, gf, gb, bw);
+    }
+    std::printf("\nworkload/rank: %lld interior cells, halo %.2f MB send + %.2f MB recv\n", info.cells,
+                info.halo_send_bytes / 1048576.0, info.halo_send_bytes / 1048576.0);
+    std::printf("bytes are algorithmic: a fused launch reads and writes every population once\n");
+    std::printf("  for one or two lattice updates; %% BW is against %.0f GB/s.\n", (double)LBM_PROF_PEAK_GBPS);
+    std::printf("PROFILE run (one synchronisation per launch) -- take throughput from SCALING.\n\n");
 }

@@ -60,10 +60,10 @@ void LBM::MPIoutput(int n)
     ComputeMacroscopic();
     double mn[4], mx[4];
     const std::vector<double> *fld[4] = {&u_h, &v_h, &w_h, &p_h};
-    for (int q = 0; q < 4; ++q) {
-        const auto r = std::minmax_element(fld[q]->begin(), fld[q]->end());
-        mn[q] = *r.first;
-        mx[q] = *r.second;
+    for (int m = 0; m < 4; ++m) {
+        const auto r = std::minmax_element(fld[m]->begin(), fld[m]->end());
+        mn[m] = *r.first;
+        mx[m] = *r.second;
     }
     const std::string path = "output" + std::to_string(n) + ".plt";
     std::FILE *fp = std::fopen(path.c_str(), "wb");
@@ -91,12 +91,12 @@ void LBM::MPIoutput(int n)
     w.i32(0);             // no auxiliary data
     w.f32(357.0f);        // end of header
     w.f32(299.0f);        // data section
-    for (int q = 0; q < 7; ++q) w.i32(2);   // every variable FP64
+    for (int m = 0; m < 7; ++m) w.i32(2);   // every variable FP64
     w.i32(0);             // no passive variables
     w.i32(0);             // no variable sharing
     w.i32(-1);            // no connectivity sharing
-    for (int q = 0; q < 3; ++q) { w.f64(0.0); w.f64(1.0); }
-    for (int q = 0; q < 4; ++q) { w.f64(mn[q]); w.f64(mx[q]); }
+    for (int m = 0; m < 3; ++m) { w.f64(0.0); w.f64(1.0); }
+    for (int m = 0; m < 4; ++m) { w.f64(mn[m]); w.f64(mx[m]); }
     if (std::ftell(fp) != 332) die("Tecplot header size");
     // 7 arrays, x fastest; fields are held (i, j, k) with k fastest
     std::vector<double> buf((size_t)glx);

@@ -158,6 +158,10 @@ typedef struct imex_info {
                                  steps run per pass)                          */
     int whole_x;              /* IMEX_TILED: 1 if tiles keep x whole and the
                                  kernels wrap it                              */
+    long long halo_send_bytes; /* IMEX_TILED: bytes of the exchange buffer
+                                 one pack fills (the 26 send buffers of the
+                                 reference, src/profile.hpp:90-93); 0 in
+                                 IMEX_WRAP                                   */
 } imex_info;
 
 /* Defaults for every field of imex_config: FP64, wrap, 256-thread
@@ -206,7 +210,8 @@ int imex_get_macro(imex_ctx *ctx, int domain, double *p, double *u, double *v,
  * drive collision and streaming from arbitrary inputs). */
 int imex_set_f(imex_ctx *ctx, int domain, const double *host);
 
-/* Wait for all work queued on the context. */
+/* Wait for all work queued on the context, including a Collision() whose
+ * Streaming() has not come yet (the counterpart of Kokkos::fence()). */
 int imex_synchronize(imex_ctx *ctx);
 
 /* Run nsteps steps between two synchronisations and report the wall time
@@ -225,14 +230,25 @@ int imex_autotune(imex_ctx *ctx, int reps, double *best_ms);
 int imex_query(imex_ctx *ctx, imex_info *info);
 
 /* Per-phase profiler, the counterpart of the reference's LBM_PROFILE build
- * (src/profile.hpp:42-113, src/profile.cpp:99-204).  With cfg.profile set,
- * each phase (first collide, fused step, bare stream, ghost fill, conserved
- * sums, macroscopic fields) is timed with HIP events.  imex_profile_read
- * gives out = {total ms, fastest ms, slowest ms, algorithmic bytes} and the
- * call count; imex_profile_phase_name returns NULL past the last phase. */
+ * (src/profile.hpp:42-113, src/profile.cpp:64-204).  With cfg.profile set,
+ * every launch is timed with HIP events and charged to a group.  Groups 0-5
+ * carry the reference's group names and byte and flop models
+ * (src/profile.cpp:64-97): "collision" (in-place collide), "pack (enq+fence)",
+ * "mpi (post+wait)" (nothing on one GPU: the unpack reads the pack buffer),
+ * "unpack (enq+fence)", "stream (enq+fence)" (bare pull-stream) and
+ * "computeMacroscopic".  Groups 6-8 have no counterpart there: the fused
+ * pull-stream + collide launch ("collision+stream (fused)", one or two
+ * lattice updates per launch), the direct ghost fill and the conserved sums.
+ * imex_profile_read gives out = {total ms, fastest ms, slowest ms, bytes,
+ * flops} summed over the calls, and the call count; bytes are algorithmic:
+ * 2 x cells x 27 x sizeof(storage) for a step launch (each population read
+ * and written once), twice the exchange-buffer bytes for pack and unpack,
+ * 27 x sizeof + 32 per cell for the macroscopic fields; flops are 594 per
+ * collided cell and 189 per cell of moments.  imex_profile_phase_name
+ * returns NULL past the last group. */
 const char *imex_profile_phase_name(int phase);
 int imex_profile_reset(imex_ctx *ctx);
-int imex_profile_read(imex_ctx *ctx, int phase, double out[4], long long *calls);
+int imex_profile_read(imex_ctx *ctx, int phase, double out[5], long long *calls);
 
 /* Measured HBM ceiling: a plain device-to-device copy of `bytes` (16 bytes
  * per lane; one or four loads per lane, plain or streaming stores: the best

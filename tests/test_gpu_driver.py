@@ -130,20 +130,96 @@ def test_driver_tecplot_output_matches_macroscopic_fields(tmp_path):
     o.close()
 
 
+def _phase_rows(out):
+    """Rows of the phase table: name -> (calls, mean, min, max, spread, imbal, % step)."""
+    rows = {}
+    for l in out.splitlines():
+        if l.startswith("STEP TOTAL") or (l.startswith("  ") and not l.startswith("  (")):
+            parts = l[36:].split()
+            if len(parts) == 7:
+                rows[l[:36].strip()] = parts
+    return rows
+
+
+def _group_rows(out):
+    """Rows of the group table: name -> (ms/step, % step, GFLOP/s, GB/s, % BW)."""
+    lines = out.splitlines()
+    k = next(i for i, l in enumerate(lines) if l.startswith("group "))
+    rows = {}
+    for l in lines[k + 2:]:
+        if not l.strip():
+            break
+        rows[l[:26].strip()] = l[26:].split()
+    return rows
+
+
 def test_driver_phase_profile(tmp_path):
     ref, _ = _run(tmp_path)
-    got, out = _run(tmp_path, IMEXLBM_PROFILE=1, IMEXLBM_MODE="tiled", IMEXLBM_TILES=8)
-    assert "=== per-phase profile: 1 GPU, 19 timed steps ===" in out
-    rows = {l[:34].strip(): l[34:].split() for l in out.splitlines()
-            if l.startswith(("fused pull-stream", "ghost fill", "conserved sums"))}
+    got, out = _run(tmp_path, IMEXLBM_PROFILE=1, IMEXLBM_MODE="tiled", IMEXLBM_TILES=8, IMEXLBM_HALO_DIRECT=0,
+                    IMEXLBM_TWO_STEP=0)
+    assert "=== per-phase profile: 1 ranks, 19 timed steps ===" in out
+    rows = _phase_rows(out)
     # the reset follows step 1's Streaming, so step 1's report is counted: 20
     # conserved reads, each filling the ghosts that the next fused step reuses
-    assert int(rows["fused pull-stream + collide"][0]) == 19
-    assert int(rows["ghost fill (pack + unpack)"][0]) == 20
-    assert int(rows["conserved sums"][0]) == 20
-    assert float(rows["fused pull-stream + collide"][5]) > 0      # GB/s
+    assert int(rows["STEP TOTAL"][0]) == 19
+    assert int(rows["collision+stream (fused)"][0]) == 19
+    assert int(rows["pack (enq+fence)"][0]) == 20 and int(rows["unpack (enq+fence)"][0]) == 20
+    assert int(rows["conserved"][0]) == 20
+    groups = _group_rows(out)
+    # the reference's group names, with rates where its model gives them
+    assert float(groups["collision+stream (fused)"][2]) > 0        # GFLOP/s
+    assert float(groups["collision+stream (fused)"][3]) > 0        # GB/s
+    assert float(groups["pack (enq+fence)"][3]) > 0
+    assert groups["pack (enq+fence)"][2] == "-"
+    assert "workload/rank: 32768 interior cells, halo 0.88 MB send + 0.88 MB recv" in out
     for a, b in zip(ref, got):
         assert abs(b[1] - a[1]) / a[1] <= 1e-12 and abs(b[5] - a[5]) / a[5] <= 1e-12
+
+
+COLL = re.compile(r"^collision \|\s+(\S+) ms \|\s+(\S+) GFLOPS$")
+
+
+def test_driver_per_step_collision_line(tmp_path):
+    # the reference's default build prints, every step, the collision time
+    # between two fences and 594 flop per cell over it (src/main.cpp:39-53)
+    perstep = os.path.join(ROOT, "imexlbm_amd", "bin", "imexlbm_perstep")
+    (tmp_path / "input.in").write_text(DECK)
+    r = subprocess.run([perstep], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    lines = [m for m in map(COLL.match, r.stdout.splitlines()) if m]
+    assert len(lines) == 20
+    for m in lines:
+        ms, gf = float(m.group(1)), float(m.group(2))
+        assert ms > 0 and gf > 0
+        # %8.4f ms and %10.2f GFLOPS of the same interval: 594 * 32^3 flop
+        assert abs(594.0 * 32 ** 3 / (ms * 1e-3) / 1e9 - gf) <= 0.01 * gf + 0.01
+    cons = [tuple(float(x) for x in m.groups()) for m in map(CONS.match, r.stdout.splitlines()) if m]
+    ref, _ = _run(tmp_path)
+    assert cons == ref
+
+
+def _parse_scaling(text):
+    """tools/parse_scaling.sh:55-66: in every line holding an MLUPS token the
+    number before it, if numeric and positive; the first interval dropped."""
+    vals = []
+    for line in text.splitlines():
+        if "MLUPS" not in line:
+            continue
+        f = line.split()
+        v = next((f[i - 1] for i in range(1, len(f)) if f[i] == "MLUPS"), "")
+        if not re.fullmatch(r"[0-9]+(\.[0-9]+)?", v) or float(v) <= 0:
+            continue
+        vals.append(float(v))
+    return vals[1:]
+
+
+def test_driver_lines_satisfy_the_scaling_parser(tmp_path):
+    deck = "1.0  0.1  1600\n0.05  40  10\n64  64  32\n"
+    cons, out = _run(tmp_path, deck)
+    vals = _parse_scaling(out)
+    assert len(vals) == 3 and all(v > 0 for v in vals)
+    # the cons lines carry no MLUPS token (src/main.cpp:83-87)
+    assert all("MLUPS" not in l for l in out.splitlines() if l.startswith("cons"))
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp16"])
@@ -164,7 +240,8 @@ def test_driver_phase_profile_two_step(tmp_path):
     deck = "1.0  0.1  1600\n0.05  40  5\n64  64  32\n"
     got, out = _run(tmp_path, deck, IMEXLBM_PROFILE=1, IMEXLBM_TWO_STEP=1)
     ref, _ = _run(tmp_path, deck, IMEXLBM_TWO_STEP=0)
-    assert "=== per-phase profile: 1 GPU, 39 timed steps ===" in out
-    rows = {l[:34].strip(): l[34:].split() for l in out.splitlines() if l.startswith("two fused steps")}
-    assert int(rows["two fused steps in one pass"][0]) >= 15
+    assert "=== per-phase profile: 1 ranks, 39 timed steps ===" in out
+    rows = _phase_rows(out)
+    # 39 steps: 15 two-step passes and one single step left over per interval
+    assert int(rows["collision+stream (fused)"][0]) >= 15
     assert got == ref
