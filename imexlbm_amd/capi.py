@@ -25,7 +25,7 @@ EXPORTS = (
     "imex_synchronize", "imex_timed_steps", "imex_autotune", "imex_query",
     "imex_destroy", "imex_device_count", "imex_strerror", "imex_last_error",
     "imex_copy_probe", "imex_get_macro", "imex_profile_phase_name", "imex_profile_reset",
-    "imex_profile_read",
+    "imex_profile_read", "imex_conserved_noalloc",
 )
 
 
@@ -92,6 +92,8 @@ def load():
         L.imex_profile_phase_name.restype = ctypes.c_char_p
         L.imex_profile_reset.argtypes = [vp]
         L.imex_profile_read.argtypes = [vp, i, dp, ctypes.POINTER(ctypes.c_longlong)]
+        L.imex_conserved_noalloc.argtypes = [vp, i, i, i, i, ctypes.c_longlong, d, vp, dp]
+        L.imex_conserved_noalloc.restype = i
         L.imex_strerror.restype = ctypes.c_char_p
         L.imex_strerror.argtypes = [i]
         L.imex_last_error.restype = ctypes.c_char_p

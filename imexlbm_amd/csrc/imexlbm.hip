@@ -21,8 +21,10 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <climits>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1130,6 +1132,55 @@ int check_domain(imex_ctx *c, int domain)
     return IMEX_OK;
 }
 
+// --------------------------------------------------------------------------
+// Scratch of the allocation-free conserved sums
+// --------------------------------------------------------------------------
+constexpr int kNoallocTile = 512;   // cells per work tile: a default context's 256 threads x 2 cells
+constexpr int kScratchDoubles = IMEX_NOALLOC_PARTIALS * 10 + 8;
+constexpr int kMaxDevices = 64;
+
+std::mutex g_scratch_mu;
+double *g_scratch_at[kMaxDevices] = {};   // device address of `scratch` per device ordinal
+
+}  // namespace
+
+// Per-workgroup partials (5 sums and their compensations) and the 5 results
+// of imex_conserved_noalloc.  Each GPU the library runs on holds its own copy
+// until the process ends.  In host code this name is not a device address:
+// only the address hipGetSymbolAddress returns for the current device is.
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+
+// The device address of `scratch` on the current device, looked up once per
+// device with that device current.
+int scratch_address(double **out)
+{
+    int dev = 0;
+    HIPC(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(IMEX_ENODEV, "device ordinal beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_at[dev]) {
+        void *p = nullptr;
+        HIPC(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(IMEX_EHIP, "hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch_at[dev] = (double *)p;
+    }
+    *out = g_scratch_at[dev];
+    return IMEX_OK;
+}
+
+template <class S>
+int launch_conserved_box_t(const void *f, const BoxDesc &b, int ntiles, int grid, double *scr, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_conserved_box<S>), dim3(grid), dim3(RED_BLOCK), 0, st, (const S *)f, b, kNoallocTile,
+                       ntiles, scr);
+    hipLaunchKernelGGL(k_conserved_final, dim3(1), dim3(RED_BLOCK), 0, st, scr, grid,
+                       scr + 10 * IMEX_NOALLOC_PARTIALS);
+    HIPC(hipGetLastError());
+    return IMEX_OK;
+}
+
 }  // namespace
 
 // ==========================================================================
@@ -1473,6 +1524,48 @@ int imex_conserved(imex_ctx *c, int domain, double out[5])
     HIPC(hipMemcpyAsync(c->h_out, c->d_out, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPC(hipStreamSynchronize(c->stream));
     std::memcpy(out, c->h_out, 5 * sizeof(double));
+    return IMEX_OK;
+}
+
+int imex_conserved_noalloc(const void *f, int precision, int nx, int ny, int nz, long long dstride, double rho0,
+                           void *stream, double out[5])
+{
+    if (!f || !out) return fail(IMEX_EINVAL, "null argument");
+    const int ss = storage_bytes(precision);
+    if (!ss) return fail(IMEX_EINVAL, "unknown precision");
+    if (nx < 1 || ny < 1 || nz < 1) return fail(IMEX_EINVAL, "extents must be positive");
+    const long long plane = (long long)nx * ny;
+    if (dstride == 0) dstride = plane;
+    if (dstride < plane) return fail(IMEX_EINVAL, "direction stride below nx*ny");
+    // the limits of the step kernels' addressing: one z-plane (27 direction
+    // planes) within 2 GiB, in-plane cell indices within 32 bits
+    if (27LL * dstride * ss >= (1LL << 31) || plane * nx >= (1LL << 32))
+        return fail(IMEX_ERANGE, "z-plane too large for 32-bit addressing (27 * dstride * sizeof must stay "
+                                 "below 2 GiB and nx * ny * nx below 2^32)");
+    const long long ntiles = (plane + kNoallocTile - 1) / kNoallocTile * nz;
+    if (ntiles > INT_MAX) return fail(IMEX_ERANGE, "more than 2^31 - 1 work tiles");
+    double *scr = nullptr;
+    TRY(scratch_address(&scr));
+    BoxDesc b{};
+    b.org = 0;
+    b.dstride = dstride;
+    b.mag = ((1ULL << 32) + (unsigned long long)nx - 1) / (unsigned long long)nx;
+    b.nx = nx; b.ny = ny; b.nz = nz;
+    b.sy = nx;
+    b.sz = (int)(27LL * dstride);
+    b.rho0 = rho0;
+    b.xnb[0] = b.xnb[1] = -1;
+    // one partial per workgroup: the grid never exceeds the scratch's rows
+    const int grid = (int)std::min<long long>(ntiles, IMEX_NOALLOC_PARTIALS);
+    hipStream_t st = (hipStream_t)stream;
+    switch (precision) {
+    case IMEX_F64: TRY(launch_conserved_box_t<double>(f, b, (int)ntiles, grid, scr, st)); break;
+    case IMEX_F32: TRY(launch_conserved_box_t<float>(f, b, (int)ntiles, grid, scr, st)); break;
+    case IMEX_F16: TRY(launch_conserved_box_t<_Float16>(f, b, (int)ntiles, grid, scr, st)); break;
+    default: TRY(launch_conserved_box_t<bf16_t>(f, b, (int)ntiles, grid, scr, st)); break;
+    }
+    HIPC(hipMemcpyAsync(out, scr + 10 * IMEX_NOALLOC_PARTIALS, 5 * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPC(hipStreamSynchronize(st));
     return IMEX_OK;
 }
 

@@ -755,39 +755,39 @@ __device__ inline void acc_merge(Acc &a, const Acc &b)
 
 constexpr int RED_BLOCK = 256;
 
+// One work tile's cells into the thread's five Neumaier sums.
 template <class S, bool WRAP, bool GATHER>
-__global__ void __launch_bounds__(RED_BLOCK)
-k_conserved(const S *__restrict__ src, const BoxDesc *__restrict__ boxes,
-            const WorkTile *__restrict__ tiles, int ntiles, int /*cells_per_thread: sets the tile size only*/, double *__restrict__ partial)
+__device__ __forceinline__ void conserved_tile(const S *__restrict__ src, const BoxDesc &d, const WorkTile &wt,
+                                               Acc acc[5])
 {
     using R = typename Store<S>::real;
-    Acc acc[5] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}};
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const WorkTile wt = tiles[t];
-        const BoxDesc &d = boxes[wt.box];
-        const S *__restrict__ sp = src + d.org + (long long)wt.z * d.sz;
-        for (int q = (int)threadIdx.x; q < wt.n; q += (int)blockDim.x) {
-            const int p = wt.p0 + q;
-            const int j = (int)(((unsigned long long)(unsigned)p * d.mag) >> 32);
-            const int i = p - j * d.nx;
-            const Nbr n = neighbours<WRAP>(d, i, j, wt.z);
-            R g[Q];
-            load_cell<S, GATHER, WRAP>(sp, d.dstride, i + j * d.sy, n, g,
-                                       std::make_integer_sequence<int, Q>{});
-            double f[Q];
+    const S *__restrict__ sp = src + d.org + (long long)wt.z * d.sz;
+    for (int q = (int)threadIdx.x; q < wt.n; q += (int)blockDim.x) {
+        const int p = wt.p0 + q;
+        const int j = (int)(((unsigned long long)(unsigned)p * d.mag) >> 32);
+        const int i = p - j * d.nx;
+        const Nbr n = neighbours<WRAP>(d, i, j, wt.z);
+        R g[Q];
+        load_cell<S, GATHER, WRAP>(sp, d.dstride, i + j * d.sy, n, g, std::make_integer_sequence<int, Q>{});
+        double f[Q];
 #pragma unroll
-            for (int ii = 0; ii < Q; ++ii) {
-                if constexpr (Store<S>::shifted) f[ii] = (double)g[ii] + weight(ii) * d.rho0;
-                else f[ii] = (double)g[ii];
-            }
-            const Moments64 m = moments64(f);
-            acc_add(acc[0], m.pl);
-            acc_add(acc[1], m.ul);
-            acc_add(acc[2], m.vl);
-            acc_add(acc[3], m.wl);
-            acc_add(acc[4], 0.5 * (m.ul * m.ul + m.vl * m.vl + m.wl * m.wl));
+        for (int ii = 0; ii < Q; ++ii) {
+            if constexpr (Store<S>::shifted) f[ii] = (double)g[ii] + weight(ii) * d.rho0;
+            else f[ii] = (double)g[ii];
         }
+        const Moments64 m = moments64(f);
+        acc_add(acc[0], m.pl);
+        acc_add(acc[1], m.ul);
+        acc_add(acc[2], m.vl);
+        acc_add(acc[3], m.wl);
+        acc_add(acc[4], 0.5 * (m.ul * m.ul + m.vl * m.vl + m.wl * m.wl));
     }
+}
+
+// The workgroup's threads merged in a fixed tree; one partial (5 sums and
+// their compensations) per workgroup.
+__device__ __forceinline__ void conserved_partial(const Acc acc[5], double *__restrict__ partial)
+{
     __shared__ double sh[2 * 5][RED_BLOCK];
     for (int k = 0; k < 5; ++k) {
         sh[2 * k][threadIdx.x] = acc[k].s;
@@ -807,6 +807,42 @@ k_conserved(const S *__restrict__ src, const BoxDesc *__restrict__ boxes,
         __syncthreads();
     }
     if (threadIdx.x < 10) partial[blockIdx.x * 10 + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+template <class S, bool WRAP, bool GATHER>
+__global__ void __launch_bounds__(RED_BLOCK)
+k_conserved(const S *__restrict__ src, const BoxDesc *__restrict__ boxes,
+            const WorkTile *__restrict__ tiles, int ntiles, int /*cells_per_thread: sets the tile size only*/, double *__restrict__ partial)
+{
+    Acc acc[5] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const WorkTile wt = tiles[t];
+        conserved_tile<S, WRAP, GATHER>(src, boxes[wt.box], wt, acc);
+    }
+    conserved_partial(acc, partial);
+}
+
+// The same sums over one periodic box of post-streaming populations whose
+// descriptor comes by value, with the work tiles of a single-domain context
+// (runs of `chunk` cells of one z-plane, in plane order) computed instead of
+// read from a table: the same cells in the same order as k_conserved over
+// that context's table, so the same bits.
+template <class S>
+__global__ void __launch_bounds__(RED_BLOCK)
+k_conserved_box(const S *__restrict__ src, const BoxDesc box, int chunk, int ntiles, double *__restrict__ partial)
+{
+    Acc acc[5] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    const int plane = box.nx * box.ny;
+    const int per_plane = (plane + chunk - 1) / chunk;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        WorkTile wt;
+        wt.box = 0;
+        wt.z = t / per_plane;
+        wt.p0 = (t - wt.z * per_plane) * chunk;
+        wt.n = min(chunk, plane - wt.p0);
+        conserved_tile<S, true, false>(src, box, wt, acc);
+    }
+    conserved_partial(acc, partial);
 }
 
 // Second stage: one workgroup folds the per-workgroup partials, in order.

@@ -196,6 +196,30 @@ int imex_step(imex_ctx *ctx, int nsteps);
  * pz, ke} of S_n, summed in compensated FP64.  Blocks. */
 int imex_conserved(imex_ctx *ctx, int domain, double out[5]);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (serving loops, harnesses that forbid hipMalloc): LBM::Conserved
+ * (src/lbm.cpp:266-306) over post-streaming populations the caller holds in
+ * device memory, with no context and no device allocation.  The partial sums
+ * live in a fixed array inside the library (IMEX_NOALLOC_PARTIALS workgroup
+ * partials of 80 bytes plus the result, 82 KB per GPU, held until the
+ * process ends); every call on one GPU uses that GPU's copy, so calls that
+ * overlap on one GPU (two host threads, or two streams) race.
+ *
+ * f: one periodic domain of nx x ny x nz cells in the library's layout,
+ * f[z][ii][y][x] with x fastest, `dstride` elements between direction
+ * planes (0 means nx*ny) and 27*dstride between z-planes; FP64 holds f,
+ * FP32/FP16/BF16 hold the shifted g = f - w_ii*rho0 (enum imex_precision).
+ * The sums are those of imex_conserved (compensated FP64, {mass, px, py, pz,
+ * ke}), accumulated in the order a default single-domain context uses (256
+ * threads and 2 cells per thread: runs of 512 cells of one z-plane), so they
+ * equal its result bit for bit.  Runs on `stream` (NULL: the null stream),
+ * then blocks until out holds the sums.  IMEX_ERANGE when one z-plane of the
+ * layout exceeds the kernels' 32-bit addressing (27*dstride*sizeof >= 2 GiB,
+ * or nx*ny*nx >= 2^32). */
+#define IMEX_NOALLOC_PARTIALS 1024
+int imex_conserved_noalloc(const void *f, int precision, int nx, int ny, int nz, long long dstride,
+                           double rho0, void *stream, double out[5]);
+
 /* Copy S_n of one domain to the host as doubles in the reference's
  * (ii, i, j, k) order, host[((ii*nx + i)*ny + j)*nz + k].  Blocks. */
 int imex_get_f(imex_ctx *ctx, int domain, double *host);
