@@ -1713,11 +1713,16 @@ int imex_timed_steps(imex_ctx *c, int nsteps, double *wall_ms, double *kernel_ms
     return IMEX_OK;
 }
 
-int imex_autotune(imex_ctx *c, int reps, double *best_ms)
+static int autotune_impl(imex_ctx *c, int reps, double *best_ms)
 {
-    TRY(check_ctx(c));
-    if (!c->initialized) return fail(IMEX_ESTATE, "imex_initialize has not been called");
     if (reps < 1) reps = 3;
+    // IMEXLBM_AUTOTUNE_FAIL_AT=k stops the tuning with an error after its
+    // k-th timed candidate (counting one-step candidates first), so that the
+    // failure path can be tested from the host
+    const char *fail_env = std::getenv("IMEXLBM_AUTOTUNE_FAIL_AT");
+    const long fail_at = fail_env && *fail_env ? std::atol(fail_env) : -1;
+    long timed = 0;
+    auto stop_here = [&]() { return fail_at >= 0 && ++timed > fail_at; };
     TRY(flush_pending(c));
     TRY(ensure_ghosts(c));
     HIPC(hipStreamSynchronize(c->stream));
@@ -1796,6 +1801,7 @@ int imex_autotune(imex_ctx *c, int reps, double *best_ms)
     for (size_t k = 0; k < cands.size(); ++k) {
         TRY(apply(cands[k]));
         TRY(time_it(reps, t[k].first));
+        if (stop_here()) return fail(IMEX_ESTATE, "autotune stopped by IMEXLBM_AUTOTUNE_FAIL_AT");
         t[k].second = (int)k;
         if (std::getenv("IMEXLBM_AUTOTUNE_LOG"))
             std::fprintf(stderr, "autotune block=%d cpt=%d xcds=%d run=%d io16=%d: %.4f ms\n", cands[k].block,
@@ -1816,6 +1822,23 @@ int imex_autotune(imex_ctx *c, int reps, double *best_ms)
     std::sort(t.begin(), t.begin() + top);
     TRY(apply(cands[t[0].second]));
     double best = t[0].first;
+    // the chosen shape as a persistent launch: that many workgroups stride
+    // over the work-tile table (a batch of uneven instances leaves a ragged
+    // last wave of workgroups that a persistent grid does not)
+    {
+        const int keep_p = c->persistent;
+        int best_p = 0;
+        for (int per_cu : {2, 4, 8}) {
+            c->persistent = per_cu * c->ncu;
+            if (c->persistent >= (int)c->tiles.size()) continue;
+            double ms = 0;
+            TRY(time_it(std::max(16, 4 * reps), ms));
+            if (std::getenv("IMEXLBM_AUTOTUNE_LOG"))
+                std::fprintf(stderr, "autotune persistent=%d: %.4f ms\n", c->persistent, ms);
+            if (ms < best) { best = ms; best_p = c->persistent; }
+        }
+        c->persistent = best_p ? best_p : keep_p;
+    }
 
     // Two steps per pass: column tile x z-planes per sweep x tile order,
     // timed per pass (two steps) against two single-step launches.
@@ -1867,6 +1890,7 @@ int imex_autotune(imex_ctx *c, int reps, double *best_ms)
         std::vector<std::pair<double, int>> t2(c2.size());
         for (size_t k = 0; k < c2.size(); ++k) {
             TRY(time2(k, reps, t2[k].first));
+            if (stop_here()) return fail(IMEX_ESTATE, "autotune stopped by IMEXLBM_AUTOTUNE_FAIL_AT");
             t2[k].second = (int)k;
         }
         // confirmation: the four fastest again over longer runs, which alone
@@ -1895,6 +1919,30 @@ int imex_autotune(imex_ctx *c, int reps, double *best_ms)
     drop_graphs(c);
     if (best_ms) *best_ms = best;
     return IMEX_OK;
+}
+
+int imex_autotune(imex_ctx *c, int reps, double *best_ms)
+{
+    TRY(check_ctx(c));
+    if (!c->initialized) return fail(IMEX_ESTATE, "imex_initialize has not been called");
+    // the launch shape in use, restored if tuning fails part way: its work
+    // tiles and two-step table are rebuilt for it, and captured graphs (which
+    // may point at tables a candidate replaced) are dropped
+    const int keep[13] = {c->block, c->cpt, c->xcds, c->xcd_run, c->io16, c->persistent, c->nt,
+                          c->two_step, c->ts_tile, c->ts_zseg, c->ts_xcds, c->ts_pf, c->ts_buf};
+    const int r = autotune_impl(c, reps, best_ms);
+    if (r == IMEX_OK) return IMEX_OK;
+    const std::string why = g_err;
+    c->block = keep[0]; c->cpt = keep[1]; c->xcds = keep[2]; c->xcd_run = keep[3]; c->io16 = keep[4];
+    c->persistent = keep[5]; c->nt = keep[6]; c->two_step = keep[7]; c->ts_tile = keep[8];
+    c->ts_zseg = keep[9]; c->ts_xcds = keep[10]; c->ts_pf = keep[11]; c->ts_buf = keep[12];
+    drop_graphs(c);
+    (void)hipStreamSynchronize(c->stream);
+    build_tiles(c);
+    const int u = upload_tiles(c);
+    c->pass_key[0] = c->pass_key[1] = -1;   // the two-step table is rebuilt on its next use
+    if (u != IMEX_OK) return u;
+    return fail(r, why + " (launch shape restored)");
 }
 
 int imex_copy_probe(int device, long long bytes, int reps, double *gbs)

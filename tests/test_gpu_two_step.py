@@ -49,13 +49,15 @@ CASES = ([("fp64", t, pf) for t in (2, 3) for pf in (0, 1)] +
          [(p, t, pf) for p in ("fp16", "bf16") for t in (0, 1, 2, 3, 4) for pf in (0, 1, 2)])
 
 
-@pytest.mark.parametrize("prec,tile,pf", CASES)
-@pytest.mark.parametrize("shape,zseg,steps", [((64, 64, 64), 0, 7), ((128, 64, 24), 7, 6),
-                                              ((64, 32, 3), 2, 5), ((64, 16, 1), 0, 4)])
+TILE_DIMS = {0: (64, 8), 1: (32, 16), 2: (32, 8), 3: (32, 8), 4: (64, 16)}
+SHAPES = [((64, 64, 64), 0, 7), ((128, 64, 24), 7, 6), ((64, 32, 3), 2, 5), ((64, 16, 1), 0, 4)]
+# every case whose column tile divides the domain
+DIVIDING = [(prec, tile, pf, shape, zseg, steps) for (shape, zseg, steps) in SHAPES for (prec, tile, pf) in CASES
+            if shape[0] % TILE_DIMS[tile][0] == 0 and shape[1] % TILE_DIMS[tile][1] == 0]
+
+
+@pytest.mark.parametrize("prec,tile,pf,shape,zseg,steps", DIVIDING)
 def test_two_step_bitwise_equals_single_steps(prec, tile, pf, shape, zseg, steps):
-    tx, ty = {0: (64, 8), 1: (32, 16), 2: (32, 8), 3: (32, 8), 4: (64, 16)}[tile]
-    if shape[0] % tx or shape[1] % ty:
-        pytest.skip("tile does not divide the domain")
     d = _dom(*shape)
     io = (tile + pf + shape[2]) % 2          # both access paths across the cases
     fa, fb, ca, cb = _pair(d, prec, steps, two_step_tile=tile, two_step_zseg=zseg, two_step_prefetch=pf,
@@ -158,6 +160,23 @@ def test_autotune_with_two_step_candidates(prec):
         assert a.autotune(reps=2) > 0
         assert np.array_equal(a.get_f(), before)
         a.step(9); b.step(9)
+        assert np.array_equal(a.get_f(), b.get_f())
+
+
+@pytest.mark.parametrize("fail_at", [3, 40])       # inside the one-step, then the two-step candidates
+def test_failed_autotune_restores_the_launch_shape(fail_at, monkeypatch):
+    from imexlbm_amd import capi
+    d = _dom(128, ny=64, nz=32)
+    with _ctx(d, precision="fp16", use_graph=1) as a, _ctx(d, precision="fp16", two_step=0) as b:
+        a.initialize(); b.initialize()
+        a.step(3); b.step(3)                      # a graph captured for the current shape
+        shape = a.info()
+        monkeypatch.setenv("IMEXLBM_AUTOTUNE_FAIL_AT", str(fail_at))
+        with pytest.raises(capi.ImexError, match="launch shape restored"):
+            a.autotune(reps=1)
+        monkeypatch.delenv("IMEXLBM_AUTOTUNE_FAIL_AT")
+        assert a.info() == shape
+        a.step(9); b.step(9)                      # the graph is captured again
         assert np.array_equal(a.get_f(), b.get_f())
 
 
@@ -341,18 +360,18 @@ def test_batched_64x16_two_step_equals_instances_alone(prec):
             assert np.array_equal(got[k], b.get_f()), k
 
 
-RAGGED = [("fp64", 3), ("fp64", 2), ("fp32", 0), ("fp32", 1), ("fp16", 0), ("fp16", 3), ("bf16", 1),
-          ("fp16", 4), ("bf16", 4)]
+RAGGED_TILES = [("fp64", 3), ("fp64", 2), ("fp32", 0), ("fp32", 1), ("fp16", 0), ("fp16", 3), ("bf16", 1),
+                ("fp16", 4), ("bf16", 4)]
+# every box at least one column tile across
+RAGGED = [(prec, tile, shape) for shape in [(100, 40, 12), (48, 72, 7), (96, 50, 5), (33, 17, 4)]
+          for (prec, tile) in RAGGED_TILES
+          if shape[0] >= TILE_DIMS[tile][0] and shape[1] >= TILE_DIMS[tile][1]]
 
 
-@pytest.mark.parametrize("prec,tile", RAGGED)
-@pytest.mark.parametrize("shape", [(100, 40, 12), (48, 72, 7), (96, 50, 5), (33, 17, 4)])
+@pytest.mark.parametrize("prec,tile,shape", RAGGED)
 def test_two_step_ragged_column_tiles(prec, tile, shape):
     # the last column tile runs past nx / ny: its ring wraps onto real cells
     # and its cells beyond the box are not stored
-    tx, ty = {0: (64, 8), 1: (32, 16), 2: (32, 8), 3: (32, 8), 4: (64, 16)}[tile]
-    if shape[0] < tx or shape[1] < ty:
-        pytest.skip("box narrower than the tile")
     d = _dom(*shape)
     for io in (0, 1):
         fa, fb, ca, cb = _pair(d, prec, 7, two_step_tile=tile, two_step_zseg=3, two_step_io=io)
