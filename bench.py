@@ -64,7 +64,7 @@ def tau0(u0, n, re):
 def shape_key(prec, n, info, mode="wrap"):
     """Key of a launch shape in the committed HBM-traffic summaries."""
     if info["two_step"]:
-        s = f"ts{info['two_step_tile']}:z{info['two_step_zseg']}"
+        s = f"ts{info['two_step_tile']}:z{info['two_step_zseg']}:x{info['two_step_xcds']}"
     else:
         s = f"step:b{info['block']}:c{info['cells_per_thread']}"
     return f"{prec}:{n}:{mode}:{s}"
@@ -309,6 +309,9 @@ def main():
                          two_step=-1, **kw)
             r["roofline"] = roofline(r, 256, float("nan"))
             r["roofline"].pop("copy_ceiling"), r["roofline"].pop("frac_of_copy")
+            if kw:
+                r["roofline"]["note"] = ("launch_ms brackets each pass together with the pack and unpack "
+                                         "ghost-fill launches before it; traffic is the pass alone")
             sub[name] = r
             print(f"bench: {name} {r['mlups']} MLUPS ({r['mlups_over_roofline']:.3f} of the roofline "
                   f"MLUPS), kernel {r['kernel_frac']:.3f} of peak", file=sys.stderr, flush=True)
