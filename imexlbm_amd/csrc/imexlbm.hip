@@ -148,6 +148,9 @@ struct imex_ctx {
     std::vector<HaloSeg> segs;
     std::vector<int2> pack_blocks, unpack_blocks, direct_blocks;
     std::vector<int2> direct_blocks_nox;   // the direct fill without the x faces
+    // the fill before a pass in side_mode: x-crossing channels into the side buffers
+    std::vector<int2> unpack_blocks_pass, direct_blocks_pass, direct_blocks_nox_pass;
+    int2 *d_unpack_pass = nullptr, *d_direct_pass = nullptr, *d_direct_nox_pass = nullptr;
     int rgrid[3] = {1, 1, 1};
 
     BoxDesc *d_boxes = nullptr;
@@ -171,6 +174,9 @@ struct imex_ctx {
     bool state_S = true;       // pool[cur] holds S (true) or P with a pull owed
     bool pending = false;      // a Collision() not yet applied
     bool ghosts_valid = false; // tiled: ghost layers of pool[cur] are current
+    bool side_mode = false;    // tiled two-step with x side buffers (add_side_buffers)
+    bool pghosts_valid = false; // side_mode: the ghosts the pass reads (side buffers and
+                                // the y/z ghosts) of pool[cur] are current
     bool xghost_ok = false;    // tiled: the x ghost columns of pool[cur] were written by
                                // the two-step pass that produced it (BoxDesc::xnb)
 
@@ -267,6 +273,7 @@ int layout_boxes(imex_ctx *c)
             const int p[3] = {me % r[0], me / r[0] / r[2], (me / r[0]) % r[2]};
             BoxDesc b{};
             b.xnb[0] = b.xnb[1] = -1;
+            b.xside[0] = b.xside[1] = -1;
             int n[3];
             for (int a = 0; a < 3; ++a) {
                 n[a] = block_len(gl[a], r[a], p[a]);
@@ -327,6 +334,25 @@ int layout_boxes(imex_ctx *c)
     }
     c->pool_elems = start;
     return IMEX_OK;
+}
+
+// The x ghost columns of the two-step pass in side buffers (side_index):
+// tiled domains split along x whose ghost tables were built for two steps
+// per pass.  The pass reads its x ghosts there, and stores its first and
+// last two columns there for its x neighbours; the ghost fill before a pass
+// writes the x-crossing channels there.  The boxes' own ghost columns stay
+// for the one-step kernels and the reads of the state, which a full fill
+// serves.  Each pool holds the side buffers of the state it carries.
+void add_side_buffers(imex_ctx *c)
+{
+    c->side_mode = c->tiled && c->halo_all && !c->whole_x;
+    if (!c->side_mode) return;
+    const long long align = 128 / c->ssize;
+    for (BoxDesc &b : c->boxes)
+        for (int sd = 0; sd < 2; ++sd) {
+            b.xside[sd] = c->pool_elems;
+            c->pool_elems += round_up(side_elems(b.ny, b.nz), 2 * align);
+        }
 }
 
 void build_tiles(imex_ctx *c)
@@ -406,8 +432,13 @@ void build_halo(imex_ctx *c)
     c->unpack_blocks.clear();
     c->direct_blocks.clear();
     c->direct_blocks_nox.clear();
+    c->unpack_blocks_pass.clear();
+    c->direct_blocks_pass.clear();
+    c->direct_blocks_nox_pass.clear();
     std::vector<HaloSeg> direct;
     std::vector<char> xface;   // direct segment of an x face (channel (+-1, 0, 0))
+    // target of each unpack / direct segment: box, x of the channel, first ghost row and plane
+    std::vector<std::array<int, 4>> utarget, dtarget;
     std::vector<std::array<int, 5>> direct_key;   // box, channel (x, y, z), layer combination
     std::map<std::array<int, 5>, int> direct_at;
     const int d = c->halo_depth;
@@ -482,6 +513,8 @@ void build_halo(imex_ctx *c)
                             ds.d_sy = us.d_sy; ds.d_sz = us.d_sz;
                             c->segs.push_back(ps);
                             unpack.push_back(us);
+                            utarget.push_back({I.box_begin + me, cx, d0[1], d0[2]});
+                            dtarget.push_back({I.box_begin + me, cx, d0[1], d0[2]});
                             direct_at[{bi, cx, cy, cz, cb}] = (int)direct.size();
                             direct_key.push_back({I.box_begin + me, -cx, -cy, -cz, cb});
                             direct.push_back(ds);
@@ -512,12 +545,43 @@ void build_halo(imex_ctx *c)
         }
     }
     c->segs.insert(c->segs.end(), direct.begin(), direct.end());
+    // With side buffers, the fill before a pass writes the channels that
+    // cross x into them: a copy of each such unpack and direct segment whose
+    // destination is the side buffer (two columns; rows and planes as in the
+    // box, ghosts included), the others shared with the full fill.
+    if (c->side_mode) {
+        const int nseg = (int)c->segs.size();
+        std::vector<int> side_copy(nseg, -1);
+        for (int s = npack; s < nseg; ++s) {
+            const std::array<int, 4> &tg = s < nsu ? utarget[s - npack] : dtarget[s - nsu];
+            if (tg[1] == 0) continue;
+            const BoxDesc &t = c->boxes[tg[0]];
+            HaloSeg sg = c->segs[s];
+            // crossing +x fills the lower side of the target, -x the upper
+            sg.d_off = side_index(t, tg[1] > 0 ? 0 : 1, 0, tg[2], tg[3], 0);
+            sg.d_ds = 2LL * (t.ny + 6);
+            sg.d_sy = 2;
+            sg.d_sz = 2 * Q * (t.ny + 6);
+            side_copy[s] = (int)c->segs.size();
+            c->segs.push_back(sg);
+        }
+        for (int s = 0; s < nseg; ++s)
+            if (side_copy[s] >= 0 && c->segs[s].partner >= 0)
+                c->segs[side_copy[s]].partner = side_copy[c->segs[s].partner];
+        auto remap = [&](const std::vector<int2> &from, std::vector<int2> &to) {
+            for (const int2 &bk : from) to.push_back({side_copy[bk.x] >= 0 ? side_copy[bk.x] : bk.x, bk.y});
+        };
+        remap(c->unpack_blocks, c->unpack_blocks_pass);
+        remap(c->direct_blocks, c->direct_blocks_pass);
+        remap(c->direct_blocks_nox, c->direct_blocks_nox_pass);
+    }
     c->xbuf_elems = xoff;
     c->halo_bytes = 0;
     c->pack_bytes = 0;
     c->direct_bytes = 0;
     c->direct_bytes_nox = 0;
-    for (int s = 0; s < (int)c->segs.size(); ++s) {   // each segment reads and writes its cells
+    // each segment reads and writes its cells (a side-buffer copy as many as its original)
+    for (int s = 0; s < nsu + (int)direct.size(); ++s) {
         const HaloSeg &sg = c->segs[s];
         const long long b = 2LL * __builtin_popcount(sg.mask) * sg.ncell * c->ssize;
         if (sg.pack == 2) {
@@ -842,19 +906,25 @@ int launch_step2(imex_ctx *c, int src, int dst)
     }
 }
 
+// The ghost fill of pool[which]: the full one (every ghost of every box), or
+// with side buffers and `pass`, the one the two-step pass reads (the side
+// buffers and the y and z ghosts).
 template <class S>
-int launch_halo_t(imex_ctx *c, int which)
+int launch_halo_t(imex_ctx *c, int which, bool pass)
 {
     S *f = (S *)pool_ptr(c, which);
     S *x = (S *)c->xbuf;
+    const bool side = pass && c->side_mode;
     if (c->halo_direct) {
         // x faces left out when the pass that produced the pool wrote them
-        const bool nox = c->xghost_ok && !c->capturing;
-        const int nd = (int)(nox ? c->direct_blocks_nox.size() : c->direct_blocks.size());
+        // (with side buffers it wrote those, which only this fill serves)
+        const bool nox = c->xghost_ok && !c->capturing && (side || !c->side_mode);
+        const std::vector<int2> &bl = side ? (nox ? c->direct_blocks_nox_pass : c->direct_blocks_pass)
+                                           : (nox ? c->direct_blocks_nox : c->direct_blocks);
+        int2 *dbl = side ? (nox ? c->d_direct_nox_pass : c->d_direct_pass) : (nox ? c->d_direct_nox : c->d_direct);
+        const int nd = (int)bl.size();
         PhaseScope ps(c, PH_DIRECT, (double)(nox ? c->direct_bytes_nox : c->direct_bytes));
-        if (nd)
-            hipLaunchKernelGGL((k_halo<S>), dim3(nd), dim3(256), 0, c->stream, f, f, c->d_segs,
-                               nox ? c->d_direct_nox : c->d_direct, nd);
+        if (nd) hipLaunchKernelGGL((k_halo<S>), dim3(nd), dim3(256), 0, c->stream, f, f, c->d_segs, dbl, nd);
         HIPC(hipGetLastError());
         return IMEX_OK;
     }
@@ -865,28 +935,35 @@ int launch_halo_t(imex_ctx *c, int which)
         HIPC(hipGetLastError());
     }
     PhaseScope ps(c, PH_UNPACK, (double)(c->halo_bytes - c->pack_bytes));
-    if (nu) hipLaunchKernelGGL((k_halo<S>), dim3(nu), dim3(256), 0, c->stream, x, f, c->d_segs, c->d_unpack, nu);
+    if (nu)
+        hipLaunchKernelGGL((k_halo<S>), dim3(nu), dim3(256), 0, c->stream, x, f, c->d_segs,
+                           side ? c->d_unpack_pass : c->d_unpack, nu);
     HIPC(hipGetLastError());
     return IMEX_OK;
 }
 
-int ensure_ghosts(imex_ctx *c)
+// Fill the ghosts of pool[cur] unless they are current: those of the
+// two-step pass (`pass`) or all of them.  Without side buffers the two are
+// the same fill.
+int ensure_ghosts(imex_ctx *c, bool pass = false)
 {
-    if (!c->tiled || c->ghosts_valid) return IMEX_OK;
+    if (!c->tiled) return IMEX_OK;
+    const bool side = pass && c->side_mode;
+    if (side ? c->pghosts_valid : c->ghosts_valid) return IMEX_OK;
     switch (c->prec) {
-    case IMEX_F64: TRY(launch_halo_t<double>(c, c->cur)); break;
-    case IMEX_F32: TRY(launch_halo_t<float>(c, c->cur)); break;
-    case IMEX_F16: TRY(launch_halo_t<_Float16>(c, c->cur)); break;
-    default: TRY(launch_halo_t<bf16_t>(c, c->cur)); break;
+    case IMEX_F64: TRY(launch_halo_t<double>(c, c->cur, side)); break;
+    case IMEX_F32: TRY(launch_halo_t<float>(c, c->cur, side)); break;
+    case IMEX_F16: TRY(launch_halo_t<_Float16>(c, c->cur, side)); break;
+    default: TRY(launch_halo_t<bf16_t>(c, c->cur, side)); break;
     }
-    c->ghosts_valid = true;
+    (side ? c->pghosts_valid : c->ghosts_valid) = true;
     return IMEX_OK;
 }
 
 int do_collide(imex_ctx *c)
 {
     TRY(launch_step(c, OP_COLLIDE, c->cur, c->cur));
-    c->ghosts_valid = false;
+    c->ghosts_valid = c->pghosts_valid = false;
     c->xghost_ok = false;
     return IMEX_OK;
 }
@@ -896,17 +973,17 @@ int do_fused(imex_ctx *c)
     TRY(ensure_ghosts(c));
     TRY(launch_step(c, OP_FUSED, c->cur, 1 - c->cur));
     c->cur = 1 - c->cur;
-    c->ghosts_valid = false;
+    c->ghosts_valid = c->pghosts_valid = false;
     c->xghost_ok = false;
     return IMEX_OK;
 }
 
 int do_fused2(imex_ctx *c)
 {
-    TRY(ensure_ghosts(c));
+    TRY(ensure_ghosts(c, true));
     TRY(launch_step2(c, c->cur, 1 - c->cur));
     c->cur = 1 - c->cur;
-    c->ghosts_valid = false;
+    c->ghosts_valid = c->pghosts_valid = false;
     c->xghost_ok = c->xdirect && !c->capturing;
     return IMEX_OK;
 }
@@ -934,7 +1011,7 @@ int do_stream(imex_ctx *c)
     TRY(ensure_ghosts(c));
     TRY(launch_step(c, OP_STREAM, c->cur, 1 - c->cur));
     c->cur = 1 - c->cur;
-    c->ghosts_valid = false;
+    c->ghosts_valid = c->pghosts_valid = false;
     c->xghost_ok = false;
     return IMEX_OK;
 }
@@ -975,9 +1052,9 @@ void drop_graphs(imex_ctx *c)
 int build_graph(imex_ctx *c, int parity)
 {
     const int save_cur = c->cur;
-    const bool save_gv = c->ghosts_valid;
+    const bool save_gv = c->ghosts_valid, save_pgv = c->pghosts_valid;
     c->cur = parity;
-    c->ghosts_valid = false;
+    c->ghosts_valid = c->pghosts_valid = false;
     c->xghost_ok = false;
     if (c->two_step) TRY(ts_prepare(c));
     HIPC(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -989,6 +1066,7 @@ int build_graph(imex_ctx *c, int parity)
     const hipError_t e = hipStreamEndCapture(c->stream, &g);
     c->cur = save_cur;
     c->ghosts_valid = save_gv;
+    c->pghosts_valid = save_pgv;
     if (r != IMEX_OK) {
         if (g) (void)hipGraphDestroy(g);
         return r;
@@ -1008,8 +1086,8 @@ int run_steps(imex_ctx *c, int nsteps)
                 if (!c->gexec[c->cur]) TRY(build_graph(c, c->cur));
                 HIPC(hipGraphLaunch(c->gexec[c->cur], c->stream));
                 if (c->gflip[c->cur]) c->cur = 1 - c->cur;
-                c->ghosts_valid = false;
-    c->xghost_ok = false;
+                c->ghosts_valid = c->pghosts_valid = false;
+                c->xghost_ok = false;
             } else if (c->two_step) {
                 TRY(do_fused2(c));
             } else {
@@ -1238,7 +1316,8 @@ void imex_destroy(imex_ctx *c)
     drop_graphs(c);
     for (void *p : {(void *)c->d_boxes, (void *)c->d_tiles, (void *)c->d_segs, (void *)c->d_pack,
                     (void *)c->d_unpack, (void *)c->d_tabs, (void *)c->d_partial, (void *)c->d_out,
-                    (void *)c->d_stage, (void *)c->d_pass, (void *)c->d_direct, (void *)c->d_direct_nox, c->pool[0],
+                    (void *)c->d_stage, (void *)c->d_pass, (void *)c->d_direct, (void *)c->d_direct_nox,
+                    (void *)c->d_unpack_pass, (void *)c->d_direct_pass, (void *)c->d_direct_nox_pass, c->pool[0],
                     c->pool[1], c->xbuf})
         if (p) (void)hipFree(p);
     if (c->h_out) (void)hipHostFree(c->h_out);
@@ -1384,6 +1463,7 @@ int imex_create(imex_ctx **out, const imex_config *cfg_in, const imex_domain *do
         c->ts_pf = ts_pf_for(c, c->ts_tile, cfg.two_step_prefetch);
         c->ts_buf = cfg.two_step_io == 0 && !c->tiled ? 1 : 0;
     }
+    add_side_buffers(c);
     if (c->tiled) build_halo(c);
 
     const size_t pool_bytes = (size_t)c->pool_elems * c->ssize;
@@ -1417,6 +1497,15 @@ int imex_create(imex_ctx **out, const imex_config *cfg_in, const imex_domain *do
             hipMemcpy(c->d_direct_nox, c->direct_blocks_nox.data(), c->direct_blocks_nox.size() * sizeof(int2),
                       hipMemcpyHostToDevice) != hipSuccess)
             return bail(fail(IMEX_EHIP, "halo table upload failed"));
+        if (c->side_mode) {
+            auto upload = [](const std::vector<int2> &v, int2 **d) {
+                return hipMalloc(d, std::max<size_t>(v.size(), 1) * sizeof(int2)) == hipSuccess &&
+                       hipMemcpy(*d, v.data(), v.size() * sizeof(int2), hipMemcpyHostToDevice) == hipSuccess;
+            };
+            if (!upload(c->unpack_blocks_pass, &c->d_unpack_pass) || !upload(c->direct_blocks_pass, &c->d_direct_pass) ||
+                !upload(c->direct_blocks_nox_pass, &c->d_direct_nox_pass))
+                return bail(fail(IMEX_EHIP, "halo table upload failed"));
+        }
     }
     const size_t bb = c->boxes.size() * sizeof(BoxDesc);
     if (hipMalloc(&c->d_boxes, bb) != hipSuccess || hipMalloc(&c->d_tabs, std::max(tab, 1) * sizeof(double)) != hipSuccess ||
@@ -1460,7 +1549,7 @@ int imex_initialize(imex_ctx *c)
     c->state_S = true;
     c->pending = false;
     c->deferred = 0;
-    c->ghosts_valid = false;
+    c->ghosts_valid = c->pghosts_valid = false;
     c->xghost_ok = false;
     HIPC(hipStreamSynchronize(c->stream));
     return IMEX_OK;
@@ -1681,7 +1770,7 @@ int imex_set_f(imex_ctx *c, int domain, const double *host)
         TRY(DISPATCH(launch_import_t, c, bi));
         HIPC(hipStreamSynchronize(c->stream));
     }
-    c->ghosts_valid = false;
+    c->ghosts_valid = c->pghosts_valid = false;
     c->xghost_ok = false;
     return IMEX_OK;
 }
@@ -1725,6 +1814,7 @@ static int autotune_impl(imex_ctx *c, int reps, double *best_ms)
     auto stop_here = [&]() { return fail_at >= 0 && ++timed > fail_at; };
     TRY(flush_pending(c));
     TRY(ensure_ghosts(c));
+    TRY(ensure_ghosts(c, true));
     HIPC(hipStreamSynchronize(c->stream));
     // Candidates: workgroup size, cells per thread (2 = paired kernel), tile
     // order over the XCDs (dispatch order, one range per XCD, runs of 4 or 64

@@ -44,7 +44,22 @@ struct BoxDesc {
     int xnb[2];         // tiled two-step pass: boxes whose two x ghost columns this
                         // box's first / last two columns fill (the lower / upper
                         // x neighbour), written by the pass itself; -1: none
+    long long xside[2]; // tiled two-step pass: element offsets of this box's
+                        // lower / upper x ghost columns in their side buffers
+                        // (side_index below); -1: the ghosts live in the rows
 };
+
+// Side buffer of the two x ghost columns on one side of a tiled box, read by
+// the two-step pass: [z][direction][y][column], ghost planes and rows
+// included (z and y from -3), so that the ghosts of a whole plane and
+// direction share a few 128-byte lines instead of one line per row.  Column
+// 0 and 1 are x = -2, -1 on the lower side and x = nx, nx + 1 on the upper.
+__host__ __device__ inline long long side_index(const BoxDesc &d, int side, int col, int y, int z, int ii)
+{
+    return d.xside[side] + (((long long)(z + 3) * Q + ii) * (d.ny + 6) + (y + 3)) * 2 + col;
+}
+
+__host__ __device__ inline long long side_elems(int ny, int nz) { return (long long)(nz + 6) * Q * (ny + 6) * 2; }
 
 struct WorkTile { int box, z, p0, n; };
 
@@ -532,6 +547,47 @@ __device__ inline void x_ghost_store(S *dst, const BoxDesc &t, int x, int y, int
                          std::make_integer_sequence<int, Q>{});
 }
 
+// Pulled population ii of a ring cell (x, y) of plane p in the outer two
+// columns of a box with side buffers: from the box when its source x - e_x
+// is inside, else from the side buffer of that side.  The address is
+// selected and loaded once: a branch per population would make each load
+// wait for the ones before it.  (Global-pointer planes only.)
+template <class S, int ii, class PL>
+__device__ inline void load_edge_one(const PL &P, int dsb, int xy, const Nbr &n, const S *pool, const BoxDesc &d,
+                                     int x, int y, int p, typename Store<S>::real &v)
+{
+    const int xs = x - EX[ii];
+    const char *box = P.template src<EZ[ii]>() + (unsigned long long)(unsigned)(ii * dsb + (xy + pull_xy<ii>(n)) * (int)sizeof(S));
+    const char *sb = reinterpret_cast<const char *>(
+        pool + side_index(d, xs < 0 ? 0 : 1, xs < 0 ? xs + 2 : xs - d.nx, y - EY[ii], p - EZ[ii], ii));
+    const char *a = (xs >= 0 && xs < d.nx) ? box : sb;
+    if constexpr (sizeof(S) == 2) v = widen16<S>(*reinterpret_cast<const unsigned short *>(a));
+    else v = *reinterpret_cast<const S *>(a);
+}
+
+template <class S, class PL, int... I>
+__device__ inline void load_cell_edge(const PL &P, int dsb, int xy, const Nbr &n, const S *pool, const BoxDesc &d,
+                                      int x, int y, int p, typename Store<S>::real (&f)[Q],
+                                      std::integer_sequence<int, I...>)
+{
+    (load_edge_one<S, I>(P, dsb, xy, n, pool, d, x, y, p, f[I]), ...);
+}
+
+// One cell's populations into column col of a box's side buffer (row y,
+// z-plane z): the tiled two-step pass storing its x neighbours' ghosts.
+// Ordinary stores: a wave writes two lanes per row here, and the rows of
+// many waves share a line, which the L2 gathers before writing it back
+// (streaming stores would send each few bytes to memory on their own).
+template <class S>
+__device__ inline void x_ghost_store_side(S *pool, const BoxDesc &t, int side, int col, int y, int z,
+                                          const typename Store<S>::real (&f)[Q])
+{
+    S *wp = pool + side_index(t, side, col, y, z, 0);
+    const long long ds = 2LL * (t.ny + 6);
+#pragma unroll
+    for (int ii = 0; ii < Q; ++ii) Store<S>::template st<false>(wp + ii * ds, f[ii]);
+}
+
 // Directions of the partial prefetch of 2-byte formats (PD): the first 11
 // populations of each thread's first ring cell.  More made no further gain at
 // 512^3; every thread owns a ring cell in the tiles that use it.
@@ -545,7 +601,6 @@ k_step2(const S *__restrict__ src, S *__restrict__ dst, const BoxDesc *__restric
     using R = typename Store<S>::real;
     using Wd = typename LdsWord<S>::type;
     constexpr int RX = TX + 2, RC = RX * (TY + 2), IC = TX * TY;
-    constexpr int NC = (RC + NTH - 1) / NTH;   // ring cells per thread
     __shared__ Wd lds[54 * RC];
     constexpr auto seq = std::make_integer_sequence<int, Q>{};
     const int w = xcd_slot((int)blockIdx.x, (int)gridDim.x, xcds, 0);
@@ -579,12 +634,47 @@ k_step2(const S *__restrict__ src, S *__restrict__ dst, const BoxDesc *__restric
         return Planes<S, GIO>(sbox + (long long)zm * d.sz, sbox + (long long)z * d.sz,
                               sbox + (long long)zp * d.sz, dbox);
     };
-    // loads the pulled populations of ring cell q of plane p into f
-    auto load_ring = [&](int p, int q, R (&f)[Q]) {
+    // Boxes with x side buffers: in a tile at the box's lower (upper) x edge
+    // the ring cells of the first (last) two columns pull some populations
+    // from the side buffers.  The ring of such a tile is walked by slots: the
+    // other cells first, row by row, then from the next wave boundary the
+    // edge-column cells, with idle slots between.  A wave then takes the
+    // side-buffer path with all its lanes or none, and never runs both.
+    const bool side = !WRAP && d.xside[0] >= 0;
+    const int elo = side && x0 == 0, ehi = side && x0 + TX >= d.nx - 1;
+    const bool edge_tile = elo || ehi;
+    const int nce = 2 * (elo + ehi);             // edge columns of the ring
+    const int nec = (RX - nce) * (TY + 2);       // ring cells clear of them
+    const int nep = (nec + 63) & ~63;            // slot of the first edge-column cell
+    constexpr int RS = WRAP ? RC : std::max((RX * (TY + 2) - 4 * (TY + 2) + 63) / 64 * 64 + 4 * (TY + 2),
+                                            ((RX - 2) * (TY + 2) + 63) / 64 * 64 + 2 * (TY + 2));   // ring slots
+    constexpr int NC = (RS + NTH - 1) / NTH;     // ring slots per thread
+    static_assert(NC == (RC + NTH - 1) / NTH, "the edge-column slots add no slot per thread");
+    // ring cell of slot s, -1 for an idle slot
+    auto ring_q = [&](int s) -> int {
+        if (!edge_tile) return s < RC ? s : -1;
+        if (s < nec) {
+            const int r = s / (RX - nce);
+            return r * RX + 2 * elo + (s - r * (RX - nce));
+        }
+        const int e = s - nep;
+        const int r = nce == 4 ? e >> 2 : e >> 1, k = nce == 4 ? e & 3 : e & 1;
+        if (e < 0 || r >= TY + 2) return -1;
+        return r * RX + (elo && k < 2 ? k : RX - 2 + k - 2 * elo);
+    };
+    // loads the pulled populations of ring cell q of plane p into f; the
+    // edge-column slots take those whose source lies beyond the box in x
+    // from the side buffers
+    auto load_ring = [&](int p, int s, int q, R (&f)[Q]) {
         int xy;
         Nbr n;
         const auto P = ring_at(p, q, xy, n);
-        load_cell_io<S, true>(P, dsb, xy, n, f, seq);
+        if constexpr (WRAP || !GIO) {
+            load_cell_io<S, true>(P, dsb, xy, n, f, seq);
+        } else {
+            if (!edge_tile || s < nep) load_cell_io<S, true>(P, dsb, xy, n, f, seq);
+            else load_cell_edge<S>(P, dsb, xy, n, src, d, x0 - 1 + q % RX, y0 - 1 + q / RX, p, f, seq);
+        }
     };
     // collide ring cell q of the intermediate plane p and keep it in LDS
     auto put_ring = [&](int p, int q, R (&f)[Q]) {
@@ -595,9 +685,11 @@ k_step2(const S *__restrict__ src, S *__restrict__ dst, const BoxDesc *__restric
     };
     // A(p) for p in [z0 - 1, z1]: plane p mod nz of the intermediate step
     auto first = [&](int p) {
-        for (int q = (int)threadIdx.x; q < RC; q += NTH) {
+        for (int s = (int)threadIdx.x; s < RS; s += NTH) {
+            const int q = ring_q(s);
+            if (q < 0) continue;
             R f[Q];
-            load_ring(p, q, f);
+            load_ring(p, s, q, f);
             put_ring(p, q, f);
         }
     };
@@ -620,9 +712,14 @@ k_step2(const S *__restrict__ src, S *__restrict__ dst, const BoxDesc *__restric
                 const int gx = x0 + tx, gy = y0 + ty;
                 if (gx < 2 && d.xnb[0] >= 0) {
                     const BoxDesc &t = boxes[d.xnb[0]];
-                    x_ghost_store<S, NT, GIO>(dst, t, t.nx + gx, gy, k, f);
+                    if (t.xside[1] >= 0) x_ghost_store_side<S>(dst, t, 1, gx, gy, k, f);
+                    else x_ghost_store<S, NT, GIO>(dst, t, t.nx + gx, gy, k, f);
                 }
-                if (gx >= d.nx - 2 && d.xnb[1] >= 0) x_ghost_store<S, NT, GIO>(dst, boxes[d.xnb[1]], gx - d.nx, gy, k, f);
+                if (gx >= d.nx - 2 && d.xnb[1] >= 0) {
+                    const BoxDesc &t = boxes[d.xnb[1]];
+                    if (t.xside[0] >= 0) x_ghost_store_side<S>(dst, t, 0, gx - d.nx + 2, gy, k, f);
+                    else x_ghost_store<S, NT, GIO>(dst, t, gx - d.nx, gy, k, f);
+                }
             }
         }
     };
@@ -630,11 +727,14 @@ k_step2(const S *__restrict__ src, S *__restrict__ dst, const BoxDesc *__restric
     first(z0);
     if constexpr (PD > 0) {
         static_assert(sizeof(S) == 2 && NTH <= RC && !PF, "partial prefetch: 2-byte formats, every thread owns a ring cell");
+        // each thread's first ring cell is clear of the outer columns
+        static_assert(WRAP || (RX - 4) * (TY + 2) >= NTH, "partial prefetch: the first ring cells take no side-buffer loads");
+        const int q1 = ring_q((int)threadIdx.x);
         unsigned short pre[PD];
         auto fetch_head = [&](int p) {
             int xy;
             Nbr n;
-            const auto P = ring_at(p, (int)threadIdx.x, xy, n);
+            const auto P = ring_at(p, q1, xy, n);
             load_cell_head_raw<PD>(P, dsb, xy, n, pre, std::make_integer_sequence<int, PD>{});
         };
         fetch_head(z0 + 1);
@@ -643,16 +743,18 @@ k_step2(const S *__restrict__ src, S *__restrict__ dst, const BoxDesc *__restric
             {
                 int xy;
                 Nbr n;
-                const auto P = ring_at(k + 1, (int)threadIdx.x, xy, n);
+                const auto P = ring_at(k + 1, q1, xy, n);
                 R f[Q];
                 load_cell_tail<S, PD>(P, dsb, xy, n, f, seq);
 #pragma unroll
                 for (int ii = 0; ii < PD; ++ii) f[ii] = widen16<S>(pre[ii]);
-                put_ring(k + 1, (int)threadIdx.x, f);
+                put_ring(k + 1, q1, f);
             }
-            for (int q = (int)threadIdx.x + NTH; q < RC; q += NTH) {
+            for (int s = (int)threadIdx.x + NTH; s < RS; s += NTH) {
+                const int q = ring_q(s);
+                if (q < 0) continue;
                 R f[Q];
-                load_ring(k + 1, q, f);
+                load_ring(k + 1, s, q, f);
                 put_ring(k + 1, q, f);
             }
             if (k + 1 < z1) fetch_head(k + 2);
@@ -662,18 +764,21 @@ k_step2(const S *__restrict__ src, S *__restrict__ dst, const BoxDesc *__restric
         }
     } else if constexpr (PF) {
         R pf[NC][Q];
+        int qs[NC];   // ring cell of each slot of this thread (-1: idle)
 #pragma unroll
-        for (int c = 0; c < NC; ++c)
-            if ((int)threadIdx.x + c * NTH < RC) load_ring(z0 + 1, (int)threadIdx.x + c * NTH, pf[c]);
+        for (int c = 0; c < NC; ++c) {
+            qs[c] = ring_q((int)threadIdx.x + c * NTH);
+            if (qs[c] >= 0) load_ring(z0 + 1, (int)threadIdx.x + c * NTH, qs[c], pf[c]);
+        }
 #pragma clang loop unroll(disable)
         for (int k = z0; k < z1; ++k) {
 #pragma unroll
             for (int c = 0; c < NC; ++c)
-                if ((int)threadIdx.x + c * NTH < RC) put_ring(k + 1, (int)threadIdx.x + c * NTH, pf[c]);
+                if (qs[c] >= 0) put_ring(k + 1, qs[c], pf[c]);
             if (k + 1 < z1) {
 #pragma unroll
                 for (int c = 0; c < NC; ++c)
-                    if ((int)threadIdx.x + c * NTH < RC) load_ring(k + 2, (int)threadIdx.x + c * NTH, pf[c]);
+                    if (qs[c] >= 0) load_ring(k + 2, (int)threadIdx.x + c * NTH, qs[c], pf[c]);
             }
             __syncthreads();
             second(k);
