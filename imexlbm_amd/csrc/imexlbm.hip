@@ -688,8 +688,10 @@ void ts_dims(int tile, int &tx, int &ty)
     ty = (tile == 1 || tile == 4) ? 16 : 8;
 }
 
-// Periodic boxes (one domain or a batch, at least one column tile across;
-// the last column may be ragged) or the tiles of one tiled domain (divided
+// Periodic boxes (one domain or a batch, at least half a column tile across
+// and one column tile deep in y; the last column may be ragged, and a box
+// narrower than the tile wraps its ring twice) or the tiles of one tiled
+// domain (divided
 // exactly by the column tile, at least two planes deep), in one-launch mode;
 // FP64 builds only the 32x8 tiles (its LDS ring needs 147 KB).
 bool ts_allowed(const imex_ctx *c, int tile)
@@ -701,7 +703,7 @@ bool ts_allowed(const imex_ctx *c, int tile)
     int tx, ty;
     ts_dims(tile, tx, ty);
     for (const BoxDesc &b : c->boxes) {
-        if (c->tiled ? (b.nx % tx || b.ny % ty || b.nz < 2) : (b.nx < tx || b.ny < ty)) return false;
+        if (c->tiled ? (b.nx % tx || b.ny % ty || b.nz < 2) : (2 * b.nx < tx || b.ny < ty)) return false;
     }
     return true;
 }
@@ -1448,9 +1450,10 @@ int imex_create(imex_ctx **out, const imex_config *cfg_in, const imex_domain *do
         }
         const bool ok = ts_allowed(c, c->ts_tile);
         if (cfg.two_step == 1 && !ok)
-            return bail(fail(IMEX_EINVAL, "two_step needs boxes whose nx and ny the column tile divides (tiled: "
-                                          "boxes at least 2 planes deep; batches: one launch for all; FP64: 32x8 "
-                                          "tiles only)"));
+            return bail(fail(IMEX_EINVAL, "two_step needs periodic boxes at least half a column tile wide and one "
+                                          "tile deep in y, or tiled boxes whose nx and ny the column tile divides, "
+                                          "at least 2 planes deep (batches: one launch for all; FP64: 32x8 tiles "
+                                          "only)"));
         c->two_step = ok && (cfg.two_step == 1 || (cfg.two_step < 0 && zdef > 0)) ? 1 : 0;
         if (c->two_step && c->tiled) {
             // the second step pulls through the first step's ring: two ghost

@@ -514,8 +514,10 @@ __device__ inline void lds_pull_cell(const typename LdsWord<S>::type *gm, const 
 struct PassBox { int wg0, box, tiles_x, ntile; };
 
 // WRAP: periodic box, ring and z-planes wrap, and the last column tile may
-// run past nx or ny (TX <= nx, TY <= ny): its ring wraps onto real cells and
-// its output cells beyond the box are not stored.  Otherwise the tile divides
+// run past nx or ny (TX <= 2 nx, TY <= ny): its ring wraps onto real cells,
+// twice in x for a box narrower than the tile (the same cell then appears
+// twice in the ring, collided identically), and its output cells beyond the
+// box are not stored.  Otherwise the tile divides
 // the box, and the ring and the planes z0 - 1 and z1 are ghost cells, filled
 // two layers deep (with the directions each layer is read for) before the
 // pass; a box that keeps x whole (wrap_x) wraps the ring in x instead.
@@ -625,6 +627,7 @@ k_step2(const S *__restrict__ src, S *__restrict__ dst, const BoxDesc *__restric
             zm = z == 0 ? d.nz - 1 : z - 1;
             zp = z == d.nz - 1 ? 0 : z + 1;
             if (x < 0) x += d.nx; else if (x >= d.nx) x -= d.nx;
+            if (d.nx < TX && x >= d.nx) x -= d.nx;   // a box narrower than the tile
             if (y < 0) y += d.ny; else if (y >= d.ny) y -= d.ny;
         } else if (d.wrap_x) {
             if (x < 0) x += d.nx; else if (x >= d.nx) x -= d.nx;

@@ -95,9 +95,11 @@ typedef struct imex_config {
                              in one pass (each workgroup sweeps z over a
                              column of cells and keeps the intermediate
                              step in LDS), 0 = one pass per step, -1 =
-                             auto (on where the domain allows it: one
-                             periodic domain, IMEX_WRAP, nx and ny
-                             multiples of the column tile)               */
+                             auto (where the domain allows it and the
+                             modelled pass time wins: periodic boxes at
+                             least half a column tile across and one
+                             tile deep in y, or tiled boxes that the
+                             column tile divides)                        */
     int two_step_tile;    /* column tile: 0 = 64x8 cells (512 threads),
                              1 = 32x16 (512), 2 = 32x8 (256), 3 = 32x8
                              (384), 4 = 64x16 (1024, 2-byte formats only);

@@ -225,14 +225,16 @@ def test_two_step_eligibility():
     d = _dom(48, ny=40, nz=8)
     with _ctx(d, precision="fp32") as c:          # 6 column tiles: too few workgroups, auto stays off
         assert c.info()["two_step"] == 0
-    with pytest.raises(capi.ImexError):           # narrower than every column tile
-        _ctx(_dom(24, ny=40, nz=8), precision="fp32", two_step=1)
+    with pytest.raises(capi.ImexError):           # narrower than half of every column tile
+        _ctx(_dom(12, ny=40, nz=8), precision="fp32", two_step=1)
+    with pytest.raises(capi.ImexError):           # the 64x8 tile over a box 31 cells across
+        _ctx(_dom(31, ny=40, nz=8), precision="fp32", two_step=1, two_step_tile=0)
     with pytest.raises(capi.ImexError):
         _ctx(_dom(64), precision="fp64", two_step=1, two_step_tile=0)   # FP64 builds only 32x8
     with pytest.raises(capi.ImexError):      # tiles of 24 x 40 cells: no column tile divides them
         _ctx(_dom(48, ny=40, nz=8), precision="fp32", two_step=1, mode=capi.TILED, ntiles=2)
-    with pytest.raises(capi.ImexError):      # a batch member of 16 cells across
-        _ctx([_dom(64), _dom(16, ny=64)], precision="fp32", two_step=1)
+    with pytest.raises(capi.ImexError):      # a batch member of 12 cells across
+        _ctx([_dom(64), _dom(12, ny=64)], precision="fp32", two_step=1)
     with pytest.raises(capi.ImexError):      # one launch per instance: no pass table
         _ctx([_dom(64), _dom(64)], precision="fp32", two_step=1, sequential=1)
     with _ctx(_dom(64, nz=1), precision="fp32", mode=capi.TILED, ntiles=2) as c:   # tiles one plane deep
@@ -362,16 +364,41 @@ def test_batched_64x16_two_step_equals_instances_alone(prec):
 
 RAGGED_TILES = [("fp64", 3), ("fp64", 2), ("fp32", 0), ("fp32", 1), ("fp16", 0), ("fp16", 3), ("bf16", 1),
                 ("fp16", 4), ("bf16", 4)]
-# every box at least one column tile across
-RAGGED = [(prec, tile, shape) for shape in [(100, 40, 12), (48, 72, 7), (96, 50, 5), (33, 17, 4)]
+
+
+@pytest.mark.parametrize("prec,tile", [("fp16", 4), ("bf16", 4), ("fp32", 0), ("fp64", 3)])
+def test_batched_narrow_instances_in_wide_column_tiles(prec, tile):
+    # a batch with instances narrower than the column tile (32 and 48 cells
+    # across for the 64-wide tiles, 16 and 24 for the 32-wide FP64 one) in
+    # one pass per two steps; each must equal its own one-step run
+    tx = TILE_DIMS[tile][0]
+    doms = [_dom(tx // 2, ny=32, nz=10, re=300), _dom(3 * tx // 4, ny=40, nz=7, re=900, rho0=1.1),
+            _dom(tx + 36, ny=24, nz=5, re=150, u0=0.03), _dom(2 * tx, ny=16, nz=12, re=600)]
+    with _ctx(doms, precision=prec, two_step=1, two_step_tile=tile, two_step_zseg=4) as a:
+        a.initialize()
+        a.step(9)
+        info = a.info()
+        assert info["two_step"] == 1 and info["two_step_tile"] == tile
+        got = [a.get_f(k) for k in range(len(doms))]
+    for k, d in enumerate(doms):
+        with _ctx(d, precision=prec, two_step=0) as b:
+            b.initialize()
+            b.step(9)
+            assert np.array_equal(got[k], b.get_f()), k
+# every box at least half a column tile across and one tile deep in y; 32
+# and 16 cells across are exactly half the 64- and 32-wide tiles, whose ring
+# then wraps twice in x
+RAGGED = [(prec, tile, shape)
+          for shape in [(100, 40, 12), (48, 72, 7), (96, 50, 5), (33, 17, 4), (32, 24, 6), (16, 40, 5)]
           for (prec, tile) in RAGGED_TILES
-          if shape[0] >= TILE_DIMS[tile][0] and shape[1] >= TILE_DIMS[tile][1]]
+          if 2 * shape[0] >= TILE_DIMS[tile][0] and shape[1] >= TILE_DIMS[tile][1]]
 
 
 @pytest.mark.parametrize("prec,tile,shape", RAGGED)
 def test_two_step_ragged_column_tiles(prec, tile, shape):
     # the last column tile runs past nx / ny: its ring wraps onto real cells
-    # and its cells beyond the box are not stored
+    # (twice in x for a box narrower than the tile) and its cells beyond the
+    # box are not stored
     d = _dom(*shape)
     for io in (0, 1):
         fa, fb, ca, cb = _pair(d, prec, 7, two_step_tile=tile, two_step_zseg=3, two_step_io=io)
