@@ -62,12 +62,14 @@ def tau0(u0, n, re):
 
 
 def shape_key(prec, n, info, mode="wrap"):
-    """Key of a launch shape in the committed HBM-traffic summaries."""
+    """Key of a launch shape in the committed HBM-traffic summaries (with the
+    padding between direction planes when there is one)."""
     if info["two_step"]:
         s = f"ts{info['two_step_tile']}:z{info['two_step_zseg']}:x{info['two_step_xcds']}"
     else:
         s = f"step:b{info['block']}:c{info['cells_per_thread']}"
-    return f"{prec}:{n}:{mode}:{s}"
+    pad = f":p{info['dir_pad']}" if info.get("dir_pad") else ""
+    return f"{prec}:{n}:{mode}:{s}{pad}"
 
 
 def traffic_from_profiles(key, n):

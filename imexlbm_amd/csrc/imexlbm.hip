@@ -1370,7 +1370,14 @@ int imex_create(imex_ctx **out, const imex_config *cfg_in, const imex_domain *do
     c->cpt = cfg.cells_per_thread ? cfg.cells_per_thread : 2;
     c->persistent = std::max(cfg.persistent, 0);
     c->nt = cfg.nontemporal >= 0 ? (cfg.nontemporal ? 1 : 0) : 1;
-    c->dir_pad = cfg.dir_pad >= 0 ? cfg.dir_pad : 0;
+    // Direction planes of a power-of-two size (256^3, 512^3, ...) start a
+    // multiple of a large power of two apart, so one row's lines of all 27
+    // directions share L2 sets: the two-step pass of 8- and 4-byte formats
+    // then fetched 1.27-1.65x its reads (the one-cell ring re-read from HBM)
+    // and 1.06-1.21x with 32 elements between planes.  2-byte formats stay
+    // unpadded (1.02-1.04x already; 32 elements would break the planes'
+    // 128-byte alignment), and tiled boxes have no power-of-two planes.
+    c->dir_pad = cfg.dir_pad >= 0 ? cfg.dir_pad : (c->ssize >= 4 && !c->tiled ? 32 : 0);
     // Launch defaults per storage width, from same-allocation A/B timings at
     // 512^3: 8- and 4-byte formats stream fastest when each XCD owns one
     // contiguous range of tiles (4-byte: runs of 64 tiles), 2-byte formats in

@@ -71,7 +71,10 @@ typedef struct imex_config {
                              workgroups stride over the work-tile table   */
     int nontemporal;      /* 1: streaming (non-temporal) stores, 0: plain
                              stores, -1 = default (streaming)             */
-    int dir_pad;          /* extra elements between direction planes; -1 = auto */
+    int dir_pad;          /* extra elements between direction planes; -1 =
+                             auto (32 for periodic FP64/FP32 contexts,
+                             which keeps one row's 27 direction lines out
+                             of a single L2 set; else 0)                 */
     int use_graph;        /* 1: imex_step replays a captured hipGraph     */
     int sequential;       /* several domains: one launch per domain per
                              step instead of one launch for all           */
