@@ -426,3 +426,26 @@ def test_two_step_fp64_128_matches_oracle_bitwise():
     o.close()
     assert np.array_equal(got, want)
     assert abs(cons[4] - wc[4]) / wc[4] < 1e-12
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "fp16"])
+def test_direction_padding_defaults_and_bit_identity(prec):
+    # periodic FP64/FP32 contexts pad their direction planes by 32 elements
+    # (the planes of a 64^3 box are a power of two apart); the padding is a
+    # layout choice only: populations are the same with and without it
+    from imexlbm_amd import capi
+    d = _dom(64, nz=20, re=800)
+    with _ctx(d, precision=prec) as a:
+        a.initialize()
+        a.step(9)
+        pad = a.info()["dir_pad"]
+        fa = a.get_f()
+    assert pad == (32 if prec in ("fp64", "fp32") else 0)
+    with _ctx(d, precision=prec, dir_pad=0 if pad else 64) as b:
+        b.initialize()
+        b.step(9)
+        assert b.info()["dir_pad"] == (0 if pad else 64)
+        fb = b.get_f()
+    assert np.array_equal(fa, fb)
+    with _ctx(d, precision=prec, mode=capi.TILED, ntiles=2) as t:   # tiled boxes: no padding
+        assert t.info()["dir_pad"] == 0
