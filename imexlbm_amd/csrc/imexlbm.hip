@@ -609,7 +609,11 @@ void build_halo(imex_ctx *c)
                 const BoxDesc &b = c->boxes[bi];
                 if (me >= nb) { c->xdirect = false; break; }
                 const BoxDesc &t = c->boxes[I.box_begin + me];
-                if (t.ny != b.ny || t.nz != b.nz || t.nx < 2 || b.nx < 2) { c->xdirect = false; break; }
+                // the pass writes into the neighbour's side buffer of that side
+                if (t.ny != b.ny || t.nz != b.nz || t.nx < 2 || b.nx < 2 || t.xside[1 - side] < 0) {
+                    c->xdirect = false;
+                    break;
+                }
                 c->boxes[bi].xnb[side] = I.box_begin + me;
             }
         }

@@ -43,7 +43,8 @@ struct BoxDesc {
     int wrap_x;         // box with ghosts in y and z only: x wraps in the kernels
     int xnb[2];         // tiled two-step pass: boxes whose two x ghost columns this
                         // box's first / last two columns fill (the lower / upper
-                        // x neighbour), written by the pass itself; -1: none
+                        // x neighbour), written by the pass itself into their
+                        // side buffers; -1: none
     long long xside[2]; // tiled two-step pass: element offsets of this box's
                         // lower / upper x ghost columns in their side buffers
                         // (side_index below); -1: the ghosts live in the rows
@@ -537,18 +538,6 @@ __device__ inline void load_cell_head_raw(const PL &P, int dsb, int xy, const Nb
     ((h[I] = P.template raw<EZ[I]>((xy + pull_xy<I>(n)) * 2, I * dsb)), ...);
 }
 
-// One cell's populations into x ghost column x (-2, -1, nx or nx + 1) of box
-// t, row y and z-plane z (interior rows and planes).
-template <class S, bool NT, bool GIO>
-__device__ inline void x_ghost_store(S *dst, const BoxDesc &t, int x, int y, int z,
-                                     const typename Store<S>::real (&f)[Q])
-{
-    S *wp = dst + t.base + t.zorg + (long long)z * t.sz;
-    const Planes<S, GIO> P(wp, wp, wp, wp);
-    store_cell_io<S, NT>(P, (int)(t.dstride * (long long)sizeof(S)), t.org_xy + x + y * t.sy, f,
-                         std::make_integer_sequence<int, Q>{});
-}
-
 // Pulled population ii of a ring cell (x, y) of plane p in the outer two
 // columns of a box with side buffers: from the box when its source x - e_x
 // is inside, else from the side buffer of that side.  The address is
@@ -713,16 +702,10 @@ k_step2(const S *__restrict__ src, S *__restrict__ dst, const BoxDesc *__restric
                 // the first and last two columns are also the x neighbours'
                 // two ghost columns: stored there too, every direction
                 const int gx = x0 + tx, gy = y0 + ty;
-                if (gx < 2 && d.xnb[0] >= 0) {
-                    const BoxDesc &t = boxes[d.xnb[0]];
-                    if (t.xside[1] >= 0) x_ghost_store_side<S>(dst, t, 1, gx, gy, k, f);
-                    else x_ghost_store<S, NT, GIO>(dst, t, t.nx + gx, gy, k, f);
-                }
-                if (gx >= d.nx - 2 && d.xnb[1] >= 0) {
-                    const BoxDesc &t = boxes[d.xnb[1]];
-                    if (t.xside[0] >= 0) x_ghost_store_side<S>(dst, t, 0, gx - d.nx + 2, gy, k, f);
-                    else x_ghost_store<S, NT, GIO>(dst, t, gx - d.nx, gy, k, f);
-                }
+                // (the host sets xnb only where the neighbour has side buffers)
+                if (gx < 2 && d.xnb[0] >= 0) x_ghost_store_side<S>(dst, boxes[d.xnb[0]], 1, gx, gy, k, f);
+                if (gx >= d.nx - 2 && d.xnb[1] >= 0)
+                    x_ghost_store_side<S>(dst, boxes[d.xnb[1]], 0, gx - d.nx + 2, gy, k, f);
             }
         }
     };
