@@ -273,6 +273,30 @@ def test_tiled_two_step_equals_wrap_single_steps(prec, ntiles, halo, direct, who
         assert abs(ca[k] - cb[k]) <= 1e-12 * abs(cb[k])
 
 
+@pytest.mark.parametrize("prec,shape,tile", [("fp64", (96, 96, 48), 3), ("fp16", (192, 96, 96), 4)])
+def test_tiled_two_step_three_boxes_across_x(prec, shape, tile):
+    # 27 tiles in a 3 x 3 x 3 split: each box's lower and upper x neighbours
+    # are different boxes, so the pass fills two different boxes' side
+    # buffers (in 2 x 2 x 2 both neighbours are the same box)
+    from imexlbm_amd import capi
+    d = _dom(*shape)
+    with _ctx(d, precision=prec, mode=capi.TILED, ntiles=27, halo_depth=3, two_step=1, two_step_tile=tile,
+              halo_direct=1) as a:
+        a.initialize()
+        a.step(9)
+        info = a.info()
+        fa, ca = a.get_f(), a.conserved()
+    assert (info["rx"], info["ry"], info["rz"]) == (3, 3, 3)
+    assert info["two_step"] == 1 and info["two_step_tile"] == tile and info["halo_direct"] == 1
+    with _ctx(d, precision=prec, two_step=0) as b:
+        b.initialize()
+        b.step(9)
+        fb, cb = b.get_f(), b.conserved()
+    assert np.array_equal(fa, fb)
+    for k in (0, 4):
+        assert abs(ca[k] - cb[k]) <= 1e-12 * abs(cb[k])
+
+
 @pytest.mark.parametrize("prec", ["fp16", "bf16"])
 @pytest.mark.parametrize("two,tile,pf", [(0, None, None), (1, 0, 2), (1, 1, 2), (1, 2, 2), (1, 4, 2),
                                          (1, 4, 0), (1, 2, 0)])
